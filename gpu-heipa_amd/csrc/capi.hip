@@ -1,0 +1,486 @@
+// extern "C" entry points (include/heipa.h).  Every call runs on the context's
+// stream, is synchronous at return, and converts internal errors to status
+// codes (the reference exits or aborts instead).
+#include "device/eval.hpp"
+#include "host/host_graph.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+using namespace heipa;
+
+struct heipa_ctx {
+    Ctx c;
+};
+
+struct heipa_host_graph {
+    HostGraph g;
+};
+
+namespace {
+
+template<typename F>
+int guarded(heipa_ctx *ctx, F &&f) {
+    if (!ctx) return HEIPA_ERR_INVALID;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const size_t fd = ctx->c.arena.front_depth(), bd = ctx->c.arena.back_depth();
+        try {
+            f(ctx->c);
+        } catch (...) {
+            // leave the arena as it was on entry
+            ctx->c.arena.release_back(bd);
+            ctx->c.arena.release_front(fd);
+            throw;
+        }
+        ctx->c.last_error.clear();
+        return HEIPA_OK;
+    } catch (const Error &e) {
+        ctx->c.last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->c.last_error = "host allocation failed";
+        return HEIPA_ERR_OOM;
+    } catch (const std::exception &e) {
+        ctx->c.last_error = e.what();
+        return HEIPA_ERR_INVALID;
+    }
+}
+
+template<typename F>
+int guarded_host(F &&f) {
+    try {
+        return f();
+    } catch (const Error &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return HEIPA_ERR_OOM;
+    } catch (...) {
+        return HEIPA_ERR_INVALID;
+    }
+}
+
+void check_graph(const heipa_graph *g) {
+    require(g != nullptr, "null graph");
+    require(g->n == 0 || (g->xadj && (g->m == 0 || g->adjncy)), "graph arrays missing");
+    require((g->flags & HEIPA_UNIFORM_EW) || g->m == 0 || g->adjwgt, "edge weights missing");
+    require((g->flags & HEIPA_UNIFORM_VW) || g->n == 0 || g->vwgt, "vertex weights missing");
+}
+
+} // namespace
+
+extern "C" {
+
+int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out) {
+    if (!out) return HEIPA_ERR_INVALID;
+    *out = nullptr;
+    heipa_ctx *ctx = new (std::nothrow) heipa_ctx();
+    if (!ctx) return HEIPA_ERR_OOM;
+    try {
+        Ctx &c = ctx->c;
+        c.device = device;
+        HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        c.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        if (arena_bytes == 0) {
+            size_t free_b = 0, total_b = 0;
+            HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+            arena_bytes = (uint64_t) (free_b / 4);
+        }
+        arena_bytes = (arena_bytes + 255) & ~(uint64_t) 255;
+        HIP_CHECK(hipMalloc(&c.arena_mem, arena_bytes));
+        c.arena.init(c.arena_mem, arena_bytes);
+        c.pinned_bytes = 1 << 16;
+        HIP_CHECK(hipHostMalloc(&c.pinned, c.pinned_bytes, hipHostMallocDefault));
+    } catch (const Error &e) {
+        std::fprintf(stderr, "heipa_ctx_create: %s\n", e.what());
+        if (ctx->c.arena_mem) (void) hipFree(ctx->c.arena_mem);
+        if (ctx->c.stream) (void) hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return e.code == HEIPA_ERR_DEVICE ? HEIPA_ERR_DEVICE : e.code;
+    }
+    *out = ctx;
+    return HEIPA_OK;
+}
+
+int heipa_ctx_destroy(heipa_ctx *ctx) {
+    if (!ctx) return HEIPA_ERR_INVALID;
+    Ctx &c = ctx->c;
+    (void) hipSetDevice(c.device);
+    if (c.stream) (void) hipStreamSynchronize(c.stream);
+    for (auto &p: c.pending) {
+        (void) hipEventDestroy(p.a);
+        (void) hipEventDestroy(p.b);
+    }
+    for (auto e: c.event_pool) (void) hipEventDestroy(e);
+    if (c.pinned) (void) hipHostFree(c.pinned);
+    if (c.arena_mem) (void) hipFree(c.arena_mem);
+    if (c.stream) (void) hipStreamDestroy(c.stream);
+    delete ctx;
+    return HEIPA_OK;
+}
+
+const char *heipa_ctx_last_error(const heipa_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+void *heipa_ctx_stream(heipa_ctx *ctx) { return ctx ? (void *) ctx->c.stream : nullptr; }
+
+uint64_t heipa_arena_mark(const heipa_ctx *ctx) { return ctx ? ctx->c.arena.front_depth() : 0; }
+
+int heipa_arena_release(heipa_ctx *ctx, uint64_t mark) {
+    return guarded(ctx, [&](Ctx &c) {
+        require(mark <= c.arena.front_depth(), "arena mark is above the current depth");
+        c.sync();
+        c.arena.release_front((size_t) mark);
+    });
+}
+
+uint64_t heipa_arena_peak(const heipa_ctx *ctx) { return ctx ? ctx->c.arena.peak() : 0; }
+
+int heipa_edge_cut(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, int64_t *cut) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(cut && (part || g->n == 0), "null argument");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        BackScope bs(c.arena);
+        GraphMeta meta = make_meta(c, d, true);
+        *cut = edge_cut(c, d, meta, part);
+    });
+}
+
+int heipa_comm_cost(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, const int32_t *D, uint32_t k,
+                    int64_t *J) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(J && D && k > 0 && (part || g->n == 0), "null argument");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        validate_part(c, part, d.n, k);
+        BackScope bs(c.arena);
+        GraphMeta meta = make_meta(c, d, true);
+        *J = comm_cost(c, d, meta, part, D, k);
+    });
+}
+
+int heipa_distance_matrix(const uint32_t *hier, const int32_t *dist, uint32_t l, int32_t *D_host) {
+    if (!hier || !dist || !D_host || l == 0) return HEIPA_ERR_INVALID;
+    u64 k = 1;
+    for (u32 i = 0; i < l; ++i) {
+        if (hier[i] == 0) return HEIPA_ERR_INVALID;
+        k *= hier[i];
+    }
+    // distance_oracle_matrix.h:64-82
+    for (u64 i = 0; i < k; ++i) {
+        for (u64 j = 0; j < k; ++j) {
+            if (i == j) {
+                D_host[i * k + j] = 0;
+                continue;
+            }
+            u32 level = 0;
+            u64 group = 1;
+            for (; level < l; ++level) {
+                group *= hier[level];
+                if (i / group == j / group) break;
+            }
+            D_host[i * k + j] = dist[level < l ? level : l - 1];
+        }
+    }
+    return HEIPA_OK;
+}
+
+int heipa_block_weights(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, uint32_t k, int64_t *bw_host,
+                        int64_t *maxw) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(k > 0 && bw_host, "bad argument");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        validate_part(c, part, d.n, k);
+        BackScope bs(c.arena);
+        s64 *bw = bs.get<s64>(k);
+        block_weights_dev(c, d, part, k, bw);
+        readback(c, (s64 *) bw_host, bw, k);
+        if (maxw) {
+            s64 m = 0;
+            for (u32 i = 0; i < k; ++i) m = std::max<s64>(m, bw_host[i]);
+            *maxw = m;
+        }
+    });
+}
+
+int heipa_match(heipa_ctx *ctx, const heipa_graph *g, uint32_t seed, uint32_t *mapping, uint32_t *nc,
+                heipa_match_stats *stats) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(nc && (mapping || g->n == 0), "null argument");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        BackScope bs(c.arena);
+        GraphMeta meta = make_meta(c, d, true);
+        *nc = match(c, d, meta, seed, mapping, stats);
+    });
+}
+
+int heipa_contract(heipa_ctx *ctx, const heipa_graph *fine, const uint32_t *mapping, uint32_t nc,
+                   heipa_graph *coarse) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(fine);
+        require(coarse && (mapping || fine->n == 0), "null argument");
+        require(nc <= fine->n, "nc exceeds n");
+        DGraph d = to_dgraph(fine);
+        validate_graph(c, d);
+        validate_part(c, mapping, d.n, nc);
+        GraphMeta meta;  // unused by contraction
+        CoarseGraph cg = contract(c, d, meta, mapping, nc);
+        c.sync();
+        coarse->n = cg.g.n;
+        coarse->m = cg.g.m;
+        coarse->g_weight = cg.g.gw;
+        coarse->flags = 0;
+        coarse->reserved = 0;
+        coarse->xadj = cg.g.xadj;
+        coarse->adjncy = cg.g.adj;
+        coarse->adjwgt = cg.g.ew;
+        coarse->vwgt = cg.g.vw;
+    });
+}
+
+int heipa_project(heipa_ctx *ctx, const uint32_t *mapping, uint32_t fine_n, const uint32_t *coarse_part,
+                  uint32_t *fine_part) {
+    return guarded(ctx, [&](Ctx &c) {
+        require(fine_n == 0 || (mapping && coarse_part && fine_part), "null argument");
+        project(c, mapping, fine_n, coarse_part, fine_part);
+        c.sync();
+    });
+}
+
+static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
+                      uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace, uint32_t trace_cap,
+                      uint32_t *trace_len) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(cut && maxw && k > 0 && (part || g->n == 0), "null argument");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        validate_part(c, part, d.n, k);
+        BackScope bs(c.arena);
+        GraphMeta meta = make_meta(c, d, true);
+        std::vector<heipa_jet_trace> tr;
+        JetResult r = jet_refine(c, d, meta, part, k, lmax, ultra != 0, level, *cut, *maxw, trace ? &tr : nullptr);
+        *cut = r.cut;
+        *maxw = r.maxw;
+        if (trace) {
+            const u32 cnt = (u32) std::min<size_t>(tr.size(), trace_cap);
+            for (u32 i = 0; i < cnt; ++i) trace[i] = tr[i];
+            if (trace_len) *trace_len = (u32) tr.size();
+        }
+    });
+}
+
+int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
+                     uint32_t level, int64_t *cut, int64_t *maxw) {
+    return jet_common(ctx, g, part, k, lmax, ultra, level, cut, maxw, nullptr, 0, nullptr);
+}
+
+int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax,
+                            int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
+                            uint32_t trace_cap, uint32_t *trace_len) {
+    return jet_common(ctx, g, part, k, lmax, ultra, level, cut, maxw, trace, trace_cap, trace_len);
+}
+
+int heipa_partition(heipa_ctx *ctx, const heipa_graph *g, uint32_t k, double imbalance, uint64_t seed, int ultra,
+                    uint32_t *part, heipa_solve_stats *stats, heipa_level_trace *levels, uint32_t level_cap) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(k > 0 && (part || g->n == 0), "bad argument");
+        require(imbalance >= 0.0, "imbalance must be non-negative");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        std::vector<heipa_level_trace> lv;
+        SolveResult r = solve(c, d, k, imbalance, seed, ultra != 0, part, levels ? &lv : nullptr);
+        if (stats) {
+            stats->cut = r.cut;
+            stats->maxw = r.maxw;
+            stats->lmax = r.lmax;
+            stats->initial_cut = r.initial_cut;
+            stats->levels = r.levels;
+            stats->jet_iterations = r.jet_iterations;
+            stats->ms_coarsening = r.ms[0];
+            stats->ms_contraction = r.ms[1];
+            stats->ms_initial = r.ms[2];
+            stats->ms_uncontraction = r.ms[3];
+            stats->ms_refinement = r.ms[4];
+        }
+        if (levels) {
+            for (u32 i = 0; i < lv.size() && i < level_cap; ++i) levels[i] = lv[i];
+        }
+    });
+}
+
+int heipa_hm_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hier, uint32_t l, double imbalance,
+                       uint64_t seed, int ultra, uint32_t *part) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(hier && l > 0 && (part || g->n == 0), "bad argument");
+        std::vector<u32> h(hier, hier + l);
+        for (u32 x: h) require(x > 0, "hierarchy entries must be positive");
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        hm_partition(c, d, h, imbalance, seed, ultra != 0, part);
+    });
+}
+
+double heipa_adaptive_imbalance(double global_imbalance, int64_t global_g_weight, uint32_t global_k,
+                                int64_t local_g_weight, uint32_t local_k_rem, uint64_t depth) {
+    return adaptive_imbalance(global_imbalance, global_g_weight, global_k, local_g_weight, local_k_rem, depth);
+}
+
+int heipa_prof_enable(heipa_ctx *ctx, int on) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.sync();
+        c.prof_on = on != 0;
+    });
+}
+
+int heipa_prof_reset(heipa_ctx *ctx) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.sync();
+        for (auto &s: c.prof_stats) s = KernelStat{};
+    });
+}
+
+uint32_t heipa_prof_count(heipa_ctx *ctx) {
+    if (!ctx) return 0;
+    try {
+        ctx->c.sync();
+    } catch (...) {
+    }
+    return (uint32_t) ctx->c.prof_names.size();
+}
+
+int heipa_prof_get(heipa_ctx *ctx, uint32_t idx, char *name, uint32_t name_cap, uint64_t *launches, double *total_ms,
+                   double *alg_bytes) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.sync();
+        require(idx < c.prof_names.size(), "profile index out of range");
+        if (name && name_cap) {
+            std::strncpy(name, c.prof_names[idx].c_str(), name_cap - 1);
+            name[name_cap - 1] = 0;
+        }
+        const KernelStat &s = c.prof_stats[idx];
+        if (launches) *launches = s.launches;
+        if (total_ms) *total_ms = s.ms;
+        if (alg_bytes) *alg_bytes = s.bytes;
+    });
+}
+
+// ---- host graph tools
+int heipa_read_metis(const char *path, heipa_host_graph **out) {
+    if (!path || !out) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        auto *h = new heipa_host_graph();
+        std::string err = read_metis(path, h->g);
+        if (!err.empty()) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            delete h;
+            return err.find("does not exist") != std::string::npos ? HEIPA_ERR_IO : HEIPA_ERR_INVALID;
+        }
+        *out = h;
+        return HEIPA_OK;
+    });
+}
+
+int heipa_write_metis(const char *path, const heipa_host_graph *g) {
+    if (!path || !g) return HEIPA_ERR_INVALID;
+    return write_metis(path, g->g).empty() ? HEIPA_OK : HEIPA_ERR_IO;
+}
+
+int heipa_write_partition(const char *path, const uint32_t *part_host, uint32_t n) {
+    if (!path || (!part_host && n)) return HEIPA_ERR_INVALID;
+    return write_partition(path, part_host, n).empty() ? HEIPA_OK : HEIPA_ERR_IO;
+}
+
+int heipa_gen_grid(uint32_t rows, uint32_t cols, heipa_host_graph **out) {
+    if (!out) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        auto *h = new heipa_host_graph();
+        h->g = gen_grid(rows, cols);
+        *out = h;
+        return HEIPA_OK;
+    });
+}
+
+int heipa_gen_rgg(uint32_t n, double avg_deg, uint64_t seed, uint32_t wmod, heipa_host_graph **out) {
+    if (!out || n == 0) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        auto *h = new heipa_host_graph();
+        h->g = gen_rgg(n, avg_deg, seed, wmod);
+        *out = h;
+        return HEIPA_OK;
+    });
+}
+
+int heipa_gen_rmat(uint32_t scale, uint32_t n, double edge_factor, double a, double b, double c, uint64_t seed,
+                   uint32_t wmod, int explicit_vw, heipa_host_graph **out) {
+    if (!out || scale == 0 || scale > 31) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        auto *h = new heipa_host_graph();
+        h->g = gen_rmat(scale, n, edge_factor, a, b, c, seed, wmod, explicit_vw != 0);
+        *out = h;
+        return HEIPA_OK;
+    });
+}
+
+int heipa_host_graph_from_arrays(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
+                                 const int32_t *vwgt, heipa_host_graph **out) {
+    if (!out || !xadj || (!adjncy && xadj[n] > 0)) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        auto *h = new heipa_host_graph();
+        HostGraph &g = h->g;
+        g.n = n;
+        g.m = xadj[n];
+        g.xadj.assign(xadj, xadj + n + 1);
+        g.adjncy.assign(adjncy, adjncy + g.m);
+        g.uniform_edge_weights = adjwgt == nullptr;
+        g.uniform_vertex_weights = vwgt == nullptr;
+        if (adjwgt) g.adjwgt.assign(adjwgt, adjwgt + g.m);
+        if (vwgt) g.vwgt.assign(vwgt, vwgt + n);
+        s64 w = 0;
+        for (u32 u = 0; u < n; ++u) w += vwgt ? vwgt[u] : 1;
+        g.g_weight = w;
+        *out = h;
+        return HEIPA_OK;
+    });
+}
+
+int heipa_host_graph_view(const heipa_host_graph *h, uint32_t *n, uint32_t *m, int64_t *g_weight, uint32_t *flags,
+                          const uint32_t **xadj, const uint32_t **adjncy, const int32_t **adjwgt,
+                          const int32_t **vwgt) {
+    if (!h) return HEIPA_ERR_INVALID;
+    const HostGraph &g = h->g;
+    if (n) *n = g.n;
+    if (m) *m = g.m;
+    if (g_weight) *g_weight = g.g_weight;
+    if (flags) *flags = (g.uniform_vertex_weights ? HEIPA_UNIFORM_VW : 0u) | (g.uniform_edge_weights ? HEIPA_UNIFORM_EW : 0u);
+    if (xadj) *xadj = g.xadj.data();
+    if (adjncy) *adjncy = g.adjncy.data();
+    if (adjwgt) *adjwgt = g.uniform_edge_weights ? nullptr : g.adjwgt.data();
+    if (vwgt) *vwgt = g.uniform_vertex_weights ? nullptr : g.vwgt.data();
+    return HEIPA_OK;
+}
+
+void heipa_host_graph_free(heipa_host_graph *g) { delete g; }
+
+int heipa_initial_partition(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
+                            const int32_t *vwgt, uint32_t k, double imbalance, uint64_t seed, uint32_t *part) {
+    if (!xadj || !part || k == 0) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        initial_partition(n, xadj, adjncy, adjwgt, vwgt, k, imbalance, seed, part);
+        return HEIPA_OK;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,273 @@
+// Device runtime shared by every kernel file: error type, the LIFO HBM arena,
+// the per-context stream and the per-kernel event timer.
+//
+// The arena keeps the reference's two-ended LIFO discipline
+// (src/datastructures/kokkos_memory_stack.h:36-187): the front stack holds
+// long-lived objects (graphs, mappings, partitions), the back stack holds
+// per-phase temporaries; both grow towards each other inside one hipMalloc.
+// Exhaustion raises Error(HEIPA_ERR_OOM) instead of abort().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/heipa.h"
+
+namespace heipa {
+
+using u8 = uint8_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+using s32 = int32_t;
+using s64 = int64_t;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                             \
+    do {                                                                                            \
+        hipError_t _e = (expr);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            throw ::heipa::Error(HEIPA_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+inline void require(bool ok, const std::string &msg) {
+    if (!ok) throw Error(HEIPA_ERR_INVALID, msg);
+}
+
+// device view of a CSR graph (heipa_graph without the C struct)
+struct DGraph {
+    u32 n = 0, m = 0;
+    s64 gw = 0;
+    const u32 *xadj = nullptr;
+    const u32 *adj = nullptr;
+    const s32 *ew = nullptr;  // null: uniform
+    const s32 *vw = nullptr;  // null: uniform
+};
+
+inline DGraph to_dgraph(const heipa_graph *g) {
+    DGraph d;
+    d.n = g->n;
+    d.m = g->m;
+    d.gw = g->g_weight;
+    d.xadj = g->xadj;
+    d.adj = g->adjncy;
+    d.ew = (g->flags & HEIPA_UNIFORM_EW) ? nullptr : g->adjwgt;
+    d.vw = (g->flags & HEIPA_UNIFORM_VW) ? nullptr : g->vwgt;
+    return d;
+}
+
+class Arena {
+public:
+    void init(void *base, size_t cap) {
+        base_ = (char *) base;
+        cap_ = cap;
+        front_ = 0;
+        back_ = cap;
+        fstack_.clear();
+        bstack_.clear();
+        peak_ = 0;
+    }
+    void *push_front(size_t bytes) {
+        size_t b = align(bytes);
+        if (front_ + b > back_) oom(bytes);
+        void *p = base_ + front_;
+        fstack_.push_back(front_);
+        front_ += b;
+        track();
+        return p;
+    }
+    void pop_front() {
+        if (fstack_.empty()) throw Error(HEIPA_ERR_INVALID, "arena: pop_front on empty stack");
+        front_ = fstack_.back();
+        fstack_.pop_back();
+    }
+    void *push_back(size_t bytes) {
+        size_t b = align(bytes);
+        if (back_ < b || back_ - b < front_) oom(bytes);
+        back_ -= b;
+        bstack_.push_back(back_ + b);
+        track();
+        return base_ + back_;
+    }
+    void pop_back() {
+        if (bstack_.empty()) throw Error(HEIPA_ERR_INVALID, "arena: pop_back on empty stack");
+        back_ = bstack_.back();
+        bstack_.pop_back();
+    }
+    size_t front_depth() const { return fstack_.size(); }
+    size_t back_depth() const { return bstack_.size(); }
+    void release_front(size_t depth) {
+        while (fstack_.size() > depth) pop_front();
+    }
+    void release_back(size_t depth) {
+        while (bstack_.size() > depth) pop_back();
+    }
+    size_t peak() const { return peak_; }
+    size_t capacity() const { return cap_; }
+    size_t in_use() const { return front_ + (cap_ - back_); }
+
+private:
+    static size_t align(size_t b) { return (b + 255) & ~(size_t) 255; }
+    void track() {
+        size_t u = in_use();
+        if (u > peak_) peak_ = u;
+    }
+    [[noreturn]] void oom(size_t bytes) const {
+        throw Error(HEIPA_ERR_OOM, "arena exhausted: requested " + std::to_string(bytes) + " bytes, " +
+                                       std::to_string(back_ - front_) + " free of " + std::to_string(cap_));
+    }
+    char *base_ = nullptr;
+    size_t cap_ = 0, front_ = 0, back_ = 0, peak_ = 0;
+    std::vector<size_t> fstack_, bstack_;
+};
+
+// Pops every back-stack chunk pushed during its lifetime.
+struct BackScope {
+    Arena &a;
+    size_t depth;
+    explicit BackScope(Arena &ar) : a(ar), depth(ar.back_depth()) {}
+    ~BackScope() { a.release_back(depth); }
+    template<typename T>
+    T *get(size_t count) { return (T *) a.push_back(sizeof(T) * (count ? count : 1)); }
+};
+
+struct KernelStat {
+    u64 launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct Ctx {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    void *arena_mem = nullptr;
+    Arena arena;
+    // pinned host staging for scalar readbacks
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    std::string last_error;
+    // profiling
+    bool prof_on = false;
+    std::vector<std::string> prof_names;
+    std::map<std::string, size_t> prof_index;
+    std::vector<KernelStat> prof_stats;
+    struct Pending {
+        size_t idx;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    size_t prof_slot(const char *name) {
+        auto it = prof_index.find(name);
+        if (it != prof_index.end()) return it->second;
+        size_t i = prof_names.size();
+        prof_names.emplace_back(name);
+        prof_stats.emplace_back();
+        prof_index.emplace(name, i);
+        return i;
+    }
+    // resolve completed timing events (call after a stream sync)
+    void prof_collect() {
+        for (auto &p: pending) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+            auto &s = prof_stats[p.idx];
+            s.launches++;
+            s.ms += ms;
+            s.bytes += p.bytes;
+            event_pool.push_back(p.a);
+            event_pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void sync() {
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (prof_on && !pending.empty()) prof_collect();
+    }
+    template<typename T>
+    T *pinned_as() { return (T *) pinned; }
+};
+
+// Kernel launch on the context stream with optional event timing.
+// `bytes` is the launch's algorithmic (compulsory) HBM traffic.
+template<typename... KArgs, typename... Args>
+inline void launch(Ctx &c, const char *name, double bytes, dim3 grid, dim3 block, size_t shmem,
+                   void (*kernel)(KArgs...), Args... args) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (c.prof_on) {
+        ea = c.get_event();
+        eb = c.get_event();
+        HIP_CHECK(hipEventRecord(ea, c.stream));
+    }
+    kernel<<<grid, block, shmem, c.stream>>>(static_cast<KArgs>(args)...);
+    HIP_CHECK(hipGetLastError());
+    if (c.prof_on) {
+        HIP_CHECK(hipEventRecord(eb, c.stream));
+        c.pending.push_back({c.prof_slot(name), ea, eb, bytes});
+        if (c.pending.size() > 4096) c.sync();
+    }
+}
+
+inline u32 div_up(u64 a, u64 b) { return (u32) ((a + b - 1) / b); }
+
+// grid size for a grid-stride loop over `items` with `per_block` items per block step
+inline u32 grid_for(const Ctx &c, u64 items, u32 per_block, u32 max_per_cu = 16) {
+    u64 g = (items + per_block - 1) / per_block;
+    u64 cap = (u64) c.num_cus * max_per_cu;
+    if (g > cap) g = cap;
+    if (g == 0) g = 1;
+    return (u32) g;
+}
+
+inline void memset_async(Ctx &c, void *p, int v, size_t bytes) {
+    if (bytes) HIP_CHECK(hipMemsetAsync(p, v, bytes, c.stream));
+}
+inline void copy_d2h(Ctx &c, void *dst, const void *src, size_t bytes) {
+    if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+}
+inline void copy_h2d(Ctx &c, void *dst, const void *src, size_t bytes) {
+    if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+}
+inline void copy_d2d(Ctx &c, void *dst, const void *src, size_t bytes) {
+    if (bytes) HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+}
+
+// read `count` values of T from device memory (synchronous)
+template<typename T>
+inline void readback(Ctx &c, T *host, const T *dev, size_t count) {
+    size_t bytes = sizeof(T) * count;
+    if (bytes <= c.pinned_bytes) {
+        copy_d2h(c, c.pinned, dev, bytes);
+        c.sync();
+        std::memcpy(host, c.pinned, bytes);
+    } else {
+        copy_d2h(c, host, dev, bytes);
+        c.sync();
+    }
+}
+
+} // namespace heipa

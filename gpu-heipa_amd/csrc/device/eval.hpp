@@ -1,0 +1,51 @@
+// Internal C++ interfaces of the device hot path (one header for all .hip files).
+#pragma once
+
+#include "graph_meta.hpp"
+
+#include <vector>
+
+namespace heipa {
+
+// ---- input validation (eval.hip); throw Error(HEIPA_ERR_INVALID)
+void validate_graph(Ctx &c, const DGraph &g);
+void validate_part(Ctx &c, const u32 *part, u32 n, u32 k);
+
+// ---- evaluators (eval.hip)
+s64 edge_cut(Ctx &c, const DGraph &g, const GraphMeta &meta, const u32 *part);
+s64 comm_cost(Ctx &c, const DGraph &g, const GraphMeta &meta, const u32 *part, const s32 *D, u32 k);
+void block_weights_dev(Ctx &c, const DGraph &g, const u32 *part, u32 k, s64 *bw_dev);
+
+// ---- matching (match.hip): mapping is a device array of n entries; returns nc
+u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping, heipa_match_stats *stats);
+
+// ---- contraction (contract.hip): the coarse arrays are pushed on the arena's
+// front stack in the order vwgt, xadj, adjncy, adjwgt.
+struct CoarseGraph {
+    DGraph g;
+    size_t front_depth_before = 0;  // release_front(front_depth_before) frees it
+};
+CoarseGraph contract(Ctx &c, const DGraph &fine, const GraphMeta &meta, const u32 *mapping, u32 nc);
+void project(Ctx &c, const u32 *mapping, u32 fine_n, const u32 *coarse_part, u32 *fine_part);
+
+// ---- Jet refinement (jet.hip)
+struct JetResult {
+    s64 cut = 0, maxw = 0;
+    u32 iterations = 0;
+};
+JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, u32 k, s64 lmax, bool ultra,
+                     u32 level, s64 cut, s64 maxw, std::vector<heipa_jet_trace> *trace);
+
+// ---- drivers (solver.hip)
+struct SolveResult {
+    s64 cut = 0, maxw = 0, lmax = 0, initial_cut = 0;
+    u32 levels = 0, jet_iterations = 0;
+    double ms[5] = {0, 0, 0, 0, 0};
+};
+SolveResult solve(Ctx &c, const DGraph &g, u32 k, double imbalance, u64 seed, bool ultra, u32 *part,
+                  std::vector<heipa_level_trace> *levels);
+void hm_partition(Ctx &c, const DGraph &g, const std::vector<u32> &hier, double imbalance, u64 seed, bool ultra,
+                  u32 *part);
+double adaptive_imbalance(double eps, s64 gW, u32 gk, s64 lW, u32 krem, u64 depth);
+
+} // namespace heipa

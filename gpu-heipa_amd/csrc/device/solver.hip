@@ -1,0 +1,353 @@
+// Multilevel drivers: the plain Solver level loop and the HM recursion.
+//
+// Solver (src/datastructures/solver.h:144-246 device ctor, :340-556): coarsen
+// while n > 8k (two-hop matching with seed 12345, then contraction), partition
+// the coarsest graph on the host, then project and Jet-refine level by level.
+// lmax = ceil((1 + eps) * W / k) in f64 (solver.h:410).
+//
+// HM (src/initial_partitioning/hierarchical_multisection.h:38-307): split with
+// k = hierarchy[level] under the adaptive imbalance of
+// global_multisection.h:39-48, extract every non-empty block as a child graph
+// (edge order preserved) and recurse; leaves write
+// offset + local block into the global partition.
+#include "eval.hpp"
+#include "../host/host_graph.hpp"
+
+#include <chrono>
+#include <cmath>
+
+namespace heipa {
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+__global__ void mapping_hash_kernel(const u32 *map, u32 n, unsigned long long *out) {
+    u64 h = 0;
+    for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64) gridDim.x * blockDim.x) {
+        u64 z = ((u64) u << 32) | map[u];
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        h += z ^ (z >> 31);
+    }
+    h = wave_sum(h);
+    if (lane_id() == 0 && h) atomicAdd(out, (unsigned long long) h);
+}
+
+u64 mapping_hash(Ctx &c, const u32 *map, u32 n) {
+    BackScope bs(c.arena);
+    auto *acc = bs.get<unsigned long long>(1);
+    fill<unsigned long long>(c, acc, 1, 0ull);
+    launch(c, "mapping_hash", 0.0, dim3(grid_for(c, n, 256)), dim3(256), 0, mapping_hash_kernel, map, n, acc);
+    unsigned long long h;
+    readback(c, &h, acc, 1);
+    return h;
+}
+
+s64 max_block_weight(Ctx &c, const DGraph &g, const u32 *part, u32 k) {
+    BackScope bs(c.arena);
+    s64 *bw = bs.get<s64>(k);
+    block_weights_dev(c, g, part, k, bw);
+    std::vector<s64> h(k);
+    readback(c, h.data(), bw, k);
+    s64 m = 0;
+    for (s64 x: h) m = std::max(m, x);
+    return m;
+}
+
+// ---- HM child extraction
+struct ChildSrc {
+    const u32 *tmp;
+    u32 id;
+    __device__ u32 operator()(u64 u) const { return tmp[u] == id; }
+};
+struct ChildDst {
+    const u32 *tmp;
+    u32 id;
+    u32 *loc;
+    u32 *c_n_to_o;
+    const u32 *n_to_o;
+    s32 *c_vw;
+    const s32 *vw;
+    __device__ void operator()(u64 u, u32 pos) const {
+        if (tmp[u] == id) {
+            loc[u] = pos;
+            c_n_to_o[pos] = n_to_o[u];
+            c_vw[pos] = vw ? vw[u] : 1;
+        }
+    }
+};
+
+__global__ void child_weight_kernel(const s32 *c_vw, u32 cn, unsigned long long *out) {
+    u64 s = 0;
+    for (u64 i = (u64) blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += (u64) gridDim.x * blockDim.x)
+        s += (u64) (s64) c_vw[i];
+    s = wave_sum(s);
+    if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long) s);
+}
+
+// per child vertex: number of neighbours inside the child (thread per vertex)
+__global__ void child_degree_kernel(DGraph g, const u32 *tmp, u32 id, const u32 *c_n_to_o_local, u32 cn, u32 *cdeg) {
+    for (u64 i = (u64) blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += (u64) gridDim.x * blockDim.x) {
+        const u32 u = c_n_to_o_local[i];
+        u32 d = 0;
+        for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) d += tmp[g.adj[j]] == id;
+        cdeg[i] = d;
+    }
+}
+
+// fill child rows in adjacency order, one wave per child vertex
+__global__ void child_fill_kernel(DGraph g, const u32 *tmp, u32 id, const u32 *loc, const u32 *local_ids, u32 cn,
+                                  const u32 *cxadj, u32 *cadj, s32 *cew) {
+    const u64 waves = (u64) gridDim.x * (blockDim.x / WAVE);
+    for (u64 i = (u64) blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; i < cn; i += waves) {
+        const u32 u = local_ids[i];
+        u32 out = cxadj[i];
+        const u32 b = g.xadj[u], e = g.xadj[u + 1];
+        for (u32 j0 = b; j0 < e; j0 += WAVE) {
+            const u32 j = j0 + lane_id();
+            bool keep = false;
+            u32 v = 0;
+            if (j < e) {
+                v = g.adj[j];
+                keep = tmp[v] == id;
+            }
+            const u64 bal = __ballot(keep);
+            if (keep) {
+                const u32 p = out + (u32) __popcll(bal & lanemask_lt());
+                cadj[p] = loc[v];
+                cew[p] = g.ew ? g.ew[j] : 1;
+            }
+            out += (u32) __popcll(bal);
+        }
+    }
+}
+
+struct LocalIdDst {
+    const u32 *tmp;
+    u32 id;
+    u32 *local_ids;
+    __device__ void operator()(u64 u, u32 pos) const {
+        if (tmp[u] == id) local_ids[pos] = (u32) u;
+    }
+};
+
+__global__ void write_leaf_kernel(const u32 *n_to_o, const u32 *tmp, u32 n, u32 offset, u32 *global) {
+    for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64) gridDim.x * blockDim.x)
+        global[n_to_o[u]] = offset + tmp[u];
+}
+
+struct HMParams {
+    std::vector<u32> hier, index_vec, k_rem;
+    double eps;
+    s64 gW;
+    u32 gk;
+    u64 seed;
+    bool ultra;
+};
+
+void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u32> &ident, const HMParams &p,
+            u32 *global) {
+    const u32 l = (u32) p.hier.size();
+    const u32 k = p.hier[level];
+    const size_t depth0 = c.arena.front_depth();
+    u32 *tmp = (u32 *) c.arena.push_front(4ull * (g.n ? g.n : 1));
+    const double imb = adaptive_imbalance(p.eps, p.gW, p.gk, g.gw, p.k_rem[l - 1 - ident.size()], l - ident.size());
+    if (k == 1) fill<u32>(c, tmp, g.n, 0u);
+    else solve(c, g, k, imb, p.seed, p.ultra, tmp, nullptr);
+    if (ident.size() == l - 1) {
+        u32 offset = 0;
+        for (u32 i = 0; i + 1 < l; ++i) offset += ident[i] * p.index_vec[p.index_vec.size() - 1 - i];
+        launch(c, "hm_write_leaf", 12.0 * g.n, dim3(grid_for(c, g.n, 256)), dim3(256), 0, write_leaf_kernel, n_to_o,
+               (const u32 *) tmp, g.n, offset, global);
+        c.arena.release_front(depth0);
+        return;
+    }
+    for (u32 id = 0; id < k; ++id) {
+        const size_t cdepth = c.arena.front_depth();
+        u32 cn = 0, cm = 0;
+        s64 cw = 0;
+        u32 *local_ids, *c_n_to_o, *cxadj, *cadj;
+        s32 *c_vw, *cew;
+        {
+            BackScope bs(c.arena);
+            u32 *loc = bs.get<u32>(g.n);
+            u32 *scal = bs.get<u32>(4);
+            // child size (compaction count) first
+            scan<u32>(c, "hm_child_index", ChildSrc{tmp, id}, LocalIdDst{tmp, id, loc}, g.n, scal);
+            readback(c, &cn, scal, 1);
+            if (cn == 0) continue;
+            c_vw = (s32 *) c.arena.push_front(4ull * cn);
+            cxadj = (u32 *) c.arena.push_front(4ull * (cn + 1));
+            c_n_to_o = (u32 *) c.arena.push_front(4ull * cn);
+            local_ids = bs.get<u32>(cn);
+            copy_d2d(c, local_ids, loc, 4ull * cn);
+            scan<u32>(c, "hm_child_index", ChildSrc{tmp, id},
+                      ChildDst{tmp, id, loc, c_n_to_o, n_to_o, c_vw, g.vw}, g.n, scal);
+            auto *acc = (unsigned long long *) bs.get<unsigned long long>(1);
+            fill<unsigned long long>(c, acc, 1, 0ull);
+            launch(c, "hm_child_weight", 4.0 * cn, dim3(grid_for(c, cn, 256)), dim3(256), 0, child_weight_kernel,
+                   (const s32 *) c_vw, cn, acc);
+            u32 *cdeg = bs.get<u32>(cn + 1);
+            launch(c, "hm_child_degree", 0.0, dim3(grid_for(c, cn, 256)), dim3(256), 0, child_degree_kernel, g,
+                   (const u32 *) tmp, id, (const u32 *) local_ids, cn, cdeg);
+            exclusive_sum<u32>(c, "hm_child_degree", cdeg, cxadj, cn);
+            unsigned long long wsum;
+            readback(c, &wsum, acc, 1);
+            cw = (s64) wsum;
+            readback(c, &cm, cxadj + cn, 1);
+            cadj = (u32 *) c.arena.push_front(4ull * (cm ? cm : 1));
+            cew = (s32 *) c.arena.push_front(4ull * (cm ? cm : 1));
+            launch(c, "hm_child_fill", 0.0, dim3(grid_for(c, cn, 4, 8)), dim3(256), 0, child_fill_kernel, g,
+                   (const u32 *) tmp, id, (const u32 *) loc, (const u32 *) local_ids, cn, (const u32 *) cxadj, cadj,
+                   cew);
+            // n_to_o of the child: global ids (ChildDst wrote n_to_o[u])
+        }
+        DGraph child;
+        child.n = cn;
+        child.m = cm;
+        child.gw = cw;
+        child.xadj = cxadj;
+        child.adj = cadj;
+        child.ew = cew;
+        child.vw = c_vw;
+        ident.push_back(id);
+        hm_rec(c, child, c_n_to_o, level - 1, ident, p, global);
+        ident.pop_back();
+        c.arena.release_front(cdepth);
+    }
+    c.arena.release_front(depth0);
+}
+
+} // namespace
+
+double adaptive_imbalance(double eps, s64 gW, u32 gk, s64 lW, u32 krem, u64 depth) {
+    double li = (1.0 + eps) * ((double) ((u64) krem * (u64) gW) / (double) ((u64) gk * (u64) lW));
+    return std::pow(li, 1.0 / (double) depth) - 1.0;
+}
+
+SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, bool ultra, u32 *part,
+                  std::vector<heipa_level_trace> *levels) {
+    SolveResult res;
+    require(k >= 1, "k must be >= 1");
+    res.lmax = (s64) std::ceil((1.0 + imbalance) * ((double) g0.gw / (double) k));
+    const size_t depth0 = c.arena.front_depth();
+    const u32 n0 = g0.n;
+    u32 *pa = (u32 *) c.arena.push_front(4ull * (n0 ? n0 : 1));
+    u32 *pb = (u32 *) c.arena.push_front(4ull * (n0 ? n0 : 1));
+    std::vector<DGraph> graphs{g0};
+    std::vector<GraphMeta> metas{make_meta(c, g0, false)};
+    std::vector<u32 *> maps;
+    const u32 max_n = 8 * k;
+    double t0 = now_ms();
+    while (graphs.back().n > max_n) {
+        const DGraph &g = graphs.back();
+        u32 *map = (u32 *) c.arena.push_front(4ull * g.n);
+        double ta = now_ms();
+        const u32 nc = match(c, g, metas.back(), 12345u, map, nullptr);
+        c.sync();
+        double tb = now_ms();
+        res.ms[0] += tb - ta;
+        CoarseGraph cg = contract(c, g, metas.back(), map, nc);
+        maps.push_back(map);
+        graphs.push_back(cg.g);
+        metas.push_back(make_meta(c, cg.g, false));
+        c.sync();
+        res.ms[1] += now_ms() - tb;
+    }
+    (void) t0;
+    // initial partition of the coarsest graph on the host
+    double ti = now_ms();
+    const DGraph &cg = graphs.back();
+    {
+        std::vector<u32> hx(cg.n + 1), ha(cg.m), hp(cg.n);
+        std::vector<s32> hw, hv;
+        readback(c, hx.data(), cg.xadj, cg.n + 1);
+        readback(c, ha.data(), cg.adj, cg.m);
+        if (cg.ew) {
+            hw.resize(cg.m);
+            readback(c, hw.data(), cg.ew, cg.m);
+        }
+        if (cg.vw) {
+            hv.resize(cg.n);
+            readback(c, hv.data(), cg.vw, cg.n);
+        }
+        initial_partition(cg.n, hx.data(), ha.data(), cg.ew ? hw.data() : nullptr, cg.vw ? hv.data() : nullptr, k,
+                          imbalance, seed, hp.data());
+        copy_h2d(c, pa, hp.data(), 4ull * cg.n);
+        c.sync();
+    }
+    s64 cut = edge_cut(c, cg, metas.back(), pa);
+    s64 maxw = max_block_weight(c, cg, pa, k);
+    res.initial_cut = cut;
+    res.ms[2] += now_ms() - ti;
+    if (levels) {
+        levels->assign(graphs.size(), heipa_level_trace{});
+        for (size_t l = 0; l < graphs.size(); ++l) {
+            auto &L = (*levels)[l];
+            L.n = graphs[l].n;
+            L.m = graphs[l].m;
+            L.nc = l + 1 < graphs.size() ? graphs[l + 1].n : 0;
+            if (l < maps.size()) L.mapping_hash = mapping_hash(c, maps[l], graphs[l].n);
+        }
+        levels->back().cut = cut;
+        levels->back().maxw = maxw;
+    }
+    u32 *cur = pa, *other = pb;
+    for (size_t l = maps.size(); l-- > 0;) {
+        double ta = now_ms();
+        project(c, maps[l], graphs[l].n, cur, other);
+        std::swap(cur, other);
+        c.sync();
+        double tb = now_ms();
+        res.ms[3] += tb - ta;
+        JetResult jr = jet_refine(c, graphs[l], metas[l], cur, k, res.lmax, ultra, (u32) l, cut, maxw, nullptr);
+        cut = jr.cut;
+        maxw = jr.maxw;
+        res.jet_iterations += jr.iterations;
+        res.ms[4] += now_ms() - tb;
+        if (levels) {
+            (*levels)[l].cut = cut;
+            (*levels)[l].maxw = maxw;
+        }
+    }
+    copy_d2d(c, part, cur, 4ull * n0);
+    c.sync();
+    res.cut = cut;
+    res.maxw = maxw;
+    res.levels = (u32) graphs.size();
+    c.arena.release_front(depth0);
+    return res;
+}
+
+void hm_partition(Ctx &c, const DGraph &g, const std::vector<u32> &hier, double imbalance, u64 seed, bool ultra,
+                  u32 *part) {
+    require(!hier.empty(), "empty hierarchy");
+    HMParams p;
+    p.hier = hier;
+    p.eps = imbalance;
+    p.gW = g.gw;
+    p.seed = seed;
+    p.ultra = ultra;
+    const u32 l = (u32) hier.size();
+    p.gk = 1;
+    for (u32 x: hier) p.gk *= x;
+    p.index_vec = {1};  // hierarchical_multisection.h:262-263
+    for (u32 i = 0; i + 1 < l; ++i) p.index_vec.push_back(p.index_vec[i] * hier[i]);
+    p.k_rem.resize(l);  // :265-270
+    u32 acc = 1;
+    for (u32 i = 0; i < l; ++i) {
+        p.k_rem[i] = acc * hier[i];
+        acc *= hier[i];
+    }
+    const size_t depth0 = c.arena.front_depth();
+    u32 *n_to_o = (u32 *) c.arena.push_front(4ull * (g.n ? g.n : 1));
+    launch(c, "hm_init", 4.0 * g.n, dim3(grid_for(c, g.n, 256)), dim3(256), 0, iota_kernel<u32>, n_to_o, (u64) g.n);
+    std::vector<u32> ident;
+    hm_rec(c, g, n_to_o, l - 1, ident, p, part);
+    c.sync();
+    c.arena.release_front(depth0);
+}
+
+} // namespace heipa

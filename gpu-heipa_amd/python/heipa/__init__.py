@@ -1,0 +1,474 @@
+"""ctypes binding of libheipa.so (include/heipa.h) for tests, the benchmark and
+Python callers.
+
+Device arrays are torch tensors on the HIP device; torch only provides the
+memory.  Every library call is synchronous at return; the wrapper
+synchronises torch's stream before a call so tensors written by torch are
+complete.  The binding fails loudly when the native library is missing: there
+is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libheipa.so")
+HEADER_PATH = os.path.abspath(os.path.join(PKG_ROOT, "..", "include", "heipa.h"))
+
+OK, ERR_OOM, ERR_INVALID, ERR_OVERFLOW, ERR_DEVICE, ERR_IO = 0, 1, 2, 3, 4, 5
+UNIFORM_VW, UNIFORM_EW = 1, 2
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+s32p = ctypes.POINTER(ctypes.c_int32)
+s64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class HeipaError(RuntimeError):
+    def __init__(self, what, code, msg=""):
+        super().__init__(f"{what} failed with status {code}: {msg}")
+        self.code = code
+
+
+class Graph(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint32), ("g_weight", ctypes.c_int64),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("xadj", ctypes.c_void_p), ("adjncy", ctypes.c_void_p),
+                ("adjwgt", ctypes.c_void_p), ("vwgt", ctypes.c_void_p)]
+
+
+class MatchStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint32) for f in
+                ("rounds", "after_hem", "after_leaf", "after_twin", "after_relative",
+                 "ran_leaf", "ran_twin", "ran_relative")]
+
+
+class JetTrace(ctypes.Structure):
+    _fields_ = [("ratio_idx", ctypes.c_uint32), ("iteration", ctypes.c_uint32), ("mode", ctypes.c_uint32),
+                ("n_cand", ctypes.c_uint32), ("n_moves", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("cut", ctypes.c_int64), ("maxw", ctypes.c_int64),
+                ("moves_hash", ctypes.c_uint64), ("gain_hash", ctypes.c_uint64)]
+
+
+class LevelTrace(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint32), ("nc", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("mapping_hash", ctypes.c_uint64), ("cut", ctypes.c_int64), ("maxw", ctypes.c_int64)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("cut", ctypes.c_int64), ("maxw", ctypes.c_int64), ("lmax", ctypes.c_int64),
+                ("initial_cut", ctypes.c_int64), ("levels", ctypes.c_uint32), ("jet_iterations", ctypes.c_uint32),
+                ("ms_coarsening", ctypes.c_double), ("ms_contraction", ctypes.c_double),
+                ("ms_initial", ctypes.c_double), ("ms_uncontraction", ctypes.c_double),
+                ("ms_refinement", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libheipa.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built; run __graft_entry__.build() or make -C gpu-heipa_amd")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, c_int, c_u32, c_u64, c_dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double
+        GP = ctypes.POINTER(Graph)
+        sig = {
+            "heipa_ctx_create": (c_int, [c_int, c_u64, ctypes.POINTER(vp)]),
+            "heipa_ctx_destroy": (c_int, [vp]),
+            "heipa_ctx_last_error": (ctypes.c_char_p, [vp]),
+            "heipa_ctx_stream": (vp, [vp]),
+            "heipa_arena_mark": (c_u64, [vp]),
+            "heipa_arena_release": (c_int, [vp, c_u64]),
+            "heipa_arena_peak": (c_u64, [vp]),
+            "heipa_edge_cut": (c_int, [vp, GP, vp, s64p]),
+            "heipa_comm_cost": (c_int, [vp, GP, vp, vp, c_u32, s64p]),
+            "heipa_distance_matrix": (c_int, [u32p, s32p, c_u32, s32p]),
+            "heipa_block_weights": (c_int, [vp, GP, vp, c_u32, s64p, s64p]),
+            "heipa_match": (c_int, [vp, GP, c_u32, vp, u32p, ctypes.POINTER(MatchStats)]),
+            "heipa_contract": (c_int, [vp, GP, vp, c_u32, GP]),
+            "heipa_project": (c_int, [vp, vp, c_u32, vp, vp]),
+            "heipa_jet_refine": (c_int, [vp, GP, vp, c_u32, ctypes.c_int64, c_int, c_u32, s64p, s64p]),
+            "heipa_jet_refine_traced": (c_int, [vp, GP, vp, c_u32, ctypes.c_int64, c_int, c_u32, s64p, s64p,
+                                                ctypes.POINTER(JetTrace), c_u32, u32p]),
+            "heipa_partition": (c_int, [vp, GP, c_u32, c_dbl, c_u64, c_int, vp, ctypes.POINTER(SolveStats),
+                                        ctypes.POINTER(LevelTrace), c_u32]),
+            "heipa_hm_partition": (c_int, [vp, GP, u32p, c_u32, c_dbl, c_u64, c_int, vp]),
+            "heipa_adaptive_imbalance": (c_dbl, [c_dbl, ctypes.c_int64, c_u32, ctypes.c_int64, c_u32, c_u64]),
+            "heipa_prof_enable": (c_int, [vp, c_int]),
+            "heipa_prof_reset": (c_int, [vp]),
+            "heipa_prof_count": (c_u32, [vp]),
+            "heipa_prof_get": (c_int, [vp, c_u32, ctypes.c_char_p, c_u32, ctypes.POINTER(c_u64),
+                                       ctypes.POINTER(c_dbl), ctypes.POINTER(c_dbl)]),
+            "heipa_read_metis": (c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
+            "heipa_write_metis": (c_int, [ctypes.c_char_p, vp]),
+            "heipa_write_partition": (c_int, [ctypes.c_char_p, u32p, c_u32]),
+            "heipa_gen_grid": (c_int, [c_u32, c_u32, ctypes.POINTER(vp)]),
+            "heipa_gen_rgg": (c_int, [c_u32, c_dbl, c_u64, c_u32, ctypes.POINTER(vp)]),
+            "heipa_gen_rmat": (c_int, [c_u32, c_u32, c_dbl, c_dbl, c_dbl, c_dbl, c_u64, c_u32, c_int,
+                                       ctypes.POINTER(vp)]),
+            "heipa_host_graph_from_arrays": (c_int, [c_u32, u32p, u32p, s32p, s32p, ctypes.POINTER(vp)]),
+            "heipa_host_graph_view": (c_int, [vp, u32p, u32p, s64p, u32p, ctypes.POINTER(u32p), ctypes.POINTER(u32p),
+                                              ctypes.POINTER(s32p), ctypes.POINTER(s32p)]),
+            "heipa_host_graph_free": (None, [vp]),
+            "heipa_initial_partition": (c_int, [c_u32, u32p, u32p, s32p, s32p, c_u32, c_dbl, c_u64, u32p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols_from_header(path=HEADER_PATH):
+    """Names of the functions include/heipa.h declares."""
+    import re
+    txt = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(heipa_\w+)\s*\(", txt, re.M)))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+# ---------------------------------------------------------------- host graphs
+@dataclass
+class HostGraph:
+    """A CSR graph in host memory (numpy arrays).  adjwgt / vwgt are None when uniform."""
+    xadj: np.ndarray
+    adjncy: np.ndarray
+    adjwgt: np.ndarray | None = None
+    vwgt: np.ndarray | None = None
+
+    @property
+    def n(self):
+        return len(self.xadj) - 1
+
+    @property
+    def m(self):
+        return int(self.xadj[-1])
+
+    @property
+    def g_weight(self):
+        return int(self.vwgt.sum()) if self.vwgt is not None else self.n
+
+    @property
+    def flags(self):
+        return (UNIFORM_VW if self.vwgt is None else 0) | (UNIFORM_EW if self.adjwgt is None else 0)
+
+
+def _from_handle(h):
+    L = lib()
+    n, m, flags = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    gw = ctypes.c_int64()
+    xa, ad = u32p(), u32p()
+    ew, vw = s32p(), s32p()
+    L.heipa_host_graph_view(h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(gw), ctypes.byref(flags),
+                            ctypes.byref(xa), ctypes.byref(ad), ctypes.byref(ew), ctypes.byref(vw))
+    N, M = n.value, m.value
+    xadj = np.ctypeslib.as_array(xa, shape=(N + 1,)).copy()
+    adjncy = np.ctypeslib.as_array(ad, shape=(M,)).copy() if M else np.zeros(0, np.uint32)
+    adjwgt = np.ctypeslib.as_array(ew, shape=(M,)).copy() if (ew and M) else (np.zeros(0, np.int32) if ew else None)
+    vwgt = np.ctypeslib.as_array(vw, shape=(N,)).copy() if (vw and N) else None
+    L.heipa_host_graph_free(h)
+    return HostGraph(xadj, adjncy, adjwgt, vwgt)
+
+
+def _gen(fn, *args):
+    h = ctypes.c_void_p()
+    rc = fn(*args, ctypes.byref(h))
+    if rc != OK:
+        raise HeipaError(fn.__name__, rc)
+    return _from_handle(h)
+
+
+def gen_grid(rows, cols):
+    return _gen(lib().heipa_gen_grid, rows, cols)
+
+
+def gen_rgg(n, avg_deg, seed, wmod=0):
+    return _gen(lib().heipa_gen_rgg, n, avg_deg, seed, wmod)
+
+
+def gen_rmat(scale, n=0, edge_factor=16.0, a=0.57, b=0.19, c=0.19, seed=1, wmod=0, explicit_vw=False):
+    return _gen(lib().heipa_gen_rmat, scale, n, edge_factor, a, b, c, seed, wmod, int(explicit_vw))
+
+
+def read_metis(path):
+    return _gen(lib().heipa_read_metis, path.encode())
+
+
+def _handle_of(g: HostGraph):
+    L = lib()
+    h = ctypes.c_void_p()
+    xadj = np.ascontiguousarray(g.xadj, np.uint32)
+    adjncy = np.ascontiguousarray(g.adjncy, np.uint32)
+    ew = None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32)
+    vw = None if g.vwgt is None else np.ascontiguousarray(g.vwgt, np.int32)
+    rc = L.heipa_host_graph_from_arrays(g.n, _ptr(xadj, ctypes.c_uint32), _ptr(adjncy, ctypes.c_uint32),
+                                        None if ew is None else _ptr(ew, ctypes.c_int32),
+                                        None if vw is None else _ptr(vw, ctypes.c_int32), ctypes.byref(h))
+    if rc != OK:
+        raise HeipaError("heipa_host_graph_from_arrays", rc)
+    return h
+
+
+def write_metis(path, g: HostGraph):
+    L = lib()
+    h = _handle_of(g)
+    rc = L.heipa_write_metis(path.encode(), h)
+    L.heipa_host_graph_free(h)
+    if rc != OK:
+        raise HeipaError("heipa_write_metis", rc)
+
+
+def write_partition(path, part):
+    part = np.ascontiguousarray(part, np.uint32)
+    rc = lib().heipa_write_partition(path.encode(), _ptr(part, ctypes.c_uint32), len(part))
+    if rc != OK:
+        raise HeipaError("heipa_write_partition", rc)
+
+
+def distance_matrix(hier, dist):
+    hier = np.ascontiguousarray(hier, np.uint32)
+    dist = np.ascontiguousarray(dist, np.int32)
+    k = int(np.prod(hier.astype(np.int64)))
+    D = np.zeros(k * k, np.int32)
+    rc = lib().heipa_distance_matrix(_ptr(hier, ctypes.c_uint32), _ptr(dist, ctypes.c_int32), len(hier),
+                                     _ptr(D, ctypes.c_int32))
+    if rc != OK:
+        raise HeipaError("heipa_distance_matrix", rc)
+    return D.reshape(k, k)
+
+
+def initial_partition(g: HostGraph, k, imbalance, seed):
+    part = np.zeros(g.n, np.uint32)
+    xadj = np.ascontiguousarray(g.xadj, np.uint32)
+    adjncy = np.ascontiguousarray(g.adjncy, np.uint32)
+    ew = None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32)
+    vw = None if g.vwgt is None else np.ascontiguousarray(g.vwgt, np.int32)
+    rc = lib().heipa_initial_partition(g.n, _ptr(xadj, ctypes.c_uint32), _ptr(adjncy, ctypes.c_uint32),
+                                       None if ew is None else _ptr(ew, ctypes.c_int32),
+                                       None if vw is None else _ptr(vw, ctypes.c_int32), k, imbalance, seed,
+                                       _ptr(part, ctypes.c_uint32))
+    if rc != OK:
+        raise HeipaError("heipa_initial_partition", rc)
+    return part
+
+
+def adaptive_imbalance(eps, gW, gk, lW, krem, depth):
+    return lib().heipa_adaptive_imbalance(eps, gW, gk, lW, krem, depth)
+
+
+# ---------------------------------------------------------------- device side
+def _torch():
+    import torch
+    return torch
+
+
+class DeviceGraph:
+    """Device copy of a HostGraph (torch tensors on cuda:N) plus its heipa_graph descriptor."""
+
+    def __init__(self, g: HostGraph, device="cuda:0"):
+        torch = _torch()
+        self.host = g
+        self.xadj = torch.from_numpy(np.ascontiguousarray(g.xadj, np.uint32).view(np.int32)).to(device)
+        self.adjncy = torch.from_numpy(np.ascontiguousarray(g.adjncy, np.uint32).view(np.int32)).to(device)
+        self.adjwgt = None if g.adjwgt is None else torch.from_numpy(np.ascontiguousarray(g.adjwgt, np.int32)).to(device)
+        self.vwgt = None if g.vwgt is None else torch.from_numpy(np.ascontiguousarray(g.vwgt, np.int32)).to(device)
+        self.desc = Graph(g.n, g.m, g.g_weight, g.flags, 0, self.xadj.data_ptr(), self.adjncy.data_ptr(),
+                          self.adjwgt.data_ptr() if self.adjwgt is not None else None,
+                          self.vwgt.data_ptr() if self.vwgt is not None else None)
+        torch.cuda.synchronize()
+
+    @property
+    def n(self):
+        return self.desc.n
+
+
+class Context:
+    """A heipa_ctx on one device (stream + LIFO arena)."""
+
+    def __init__(self, device=0, arena_bytes=0):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        rc = self.L.heipa_ctx_create(device, arena_bytes, ctypes.byref(self.h))
+        if rc != OK:
+            raise HeipaError("heipa_ctx_create", rc)
+
+    def close(self):
+        if self.h:
+            self.L.heipa_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, name, rc):
+        if rc != OK:
+            raise HeipaError(name, rc, self.L.heipa_ctx_last_error(self.h).decode())
+
+    @staticmethod
+    def _sync():
+        _torch().cuda.synchronize()
+
+    def edge_cut(self, g: DeviceGraph, part):
+        self._sync()
+        out = ctypes.c_int64()
+        self._check("heipa_edge_cut", self.L.heipa_edge_cut(self.h, ctypes.byref(g.desc), part.data_ptr(), ctypes.byref(out)))
+        return out.value
+
+    def comm_cost(self, g: DeviceGraph, part, D, k):
+        self._sync()
+        out = ctypes.c_int64()
+        self._check("heipa_comm_cost", self.L.heipa_comm_cost(self.h, ctypes.byref(g.desc), part.data_ptr(),
+                                                              D.data_ptr(), k, ctypes.byref(out)))
+        return out.value
+
+    def block_weights(self, g: DeviceGraph, part, k):
+        self._sync()
+        bw = np.zeros(k, np.int64)
+        mx = ctypes.c_int64()
+        self._check("heipa_block_weights", self.L.heipa_block_weights(self.h, ctypes.byref(g.desc), part.data_ptr(), k,
+                                                                      _ptr(bw, ctypes.c_int64), ctypes.byref(mx)))
+        return bw, mx.value
+
+    def match(self, g: DeviceGraph, seed=12345):
+        torch = _torch()
+        mapping = torch.empty(max(g.n, 1), dtype=torch.int32, device=g.xadj.device)
+        nc = ctypes.c_uint32()
+        st = MatchStats()
+        self._sync()
+        self._check("heipa_match", self.L.heipa_match(self.h, ctypes.byref(g.desc), seed, mapping.data_ptr(),
+                                                      ctypes.byref(nc), ctypes.byref(st)))
+        return mapping[:g.n], nc.value, st
+
+    def contract(self, g: DeviceGraph, mapping, nc):
+        """Returns (HostGraph of the coarse graph); the device copy is released."""
+        self._sync()
+        mark = self.L.heipa_arena_mark(self.h)
+        out = Graph()
+        self._check("heipa_contract", self.L.heipa_contract(self.h, ctypes.byref(g.desc), mapping.data_ptr(), nc,
+                                                            ctypes.byref(out)))
+        hg = _download_graph(out)
+        self._check("heipa_arena_release", self.L.heipa_arena_release(self.h, mark))
+        return hg
+
+    def project(self, mapping, coarse_part, fine_n):
+        torch = _torch()
+        fine = torch.empty(max(fine_n, 1), dtype=torch.int32, device=coarse_part.device)
+        self._sync()
+        self._check("heipa_project", self.L.heipa_project(self.h, mapping.data_ptr(), fine_n, coarse_part.data_ptr(),
+                                                          fine.data_ptr()))
+        return fine[:fine_n]
+
+    def jet_refine(self, g: DeviceGraph, part, k, lmax, ultra, level, cut, maxw, trace_cap=0):
+        self._sync()
+        c, mw = ctypes.c_int64(cut), ctypes.c_int64(maxw)
+        if trace_cap:
+            tr = (JetTrace * trace_cap)()
+            tl = ctypes.c_uint32()
+            self._check("heipa_jet_refine_traced", self.L.heipa_jet_refine_traced(
+                self.h, ctypes.byref(g.desc), part.data_ptr(), k, lmax, int(ultra), level, ctypes.byref(c),
+                ctypes.byref(mw), tr, trace_cap, ctypes.byref(tl)))
+            return c.value, mw.value, [tr[i] for i in range(min(tl.value, trace_cap))]
+        self._check("heipa_jet_refine", self.L.heipa_jet_refine(self.h, ctypes.byref(g.desc), part.data_ptr(), k, lmax,
+                                                                int(ultra), level, ctypes.byref(c), ctypes.byref(mw)))
+        return c.value, mw.value, None
+
+    def partition(self, g: DeviceGraph, k, imbalance=0.03, seed=1, ultra=False, part=None, level_cap=0):
+        torch = _torch()
+        if part is None:
+            part = torch.empty(max(g.n, 1), dtype=torch.int32, device=g.xadj.device)
+        st = SolveStats()
+        lv = (LevelTrace * level_cap)() if level_cap else None
+        self._sync()
+        self._check("heipa_partition", self.L.heipa_partition(self.h, ctypes.byref(g.desc), k, imbalance, seed,
+                                                              int(ultra), part.data_ptr(), ctypes.byref(st), lv,
+                                                              level_cap))
+        levels = [lv[i] for i in range(min(st.levels, level_cap))] if level_cap else None
+        return part[:g.n], st, levels
+
+    def hm_partition(self, g: DeviceGraph, hier, imbalance=0.03, seed=1, ultra=False, part=None):
+        torch = _torch()
+        if part is None:
+            part = torch.empty(max(g.n, 1), dtype=torch.int32, device=g.xadj.device)
+        h = np.ascontiguousarray(hier, np.uint32)
+        self._sync()
+        self._check("heipa_hm_partition", self.L.heipa_hm_partition(self.h, ctypes.byref(g.desc),
+                                                                    _ptr(h, ctypes.c_uint32), len(h), imbalance, seed,
+                                                                    int(ultra), part.data_ptr()))
+        return part[:g.n]
+
+    # per-kernel timing
+    def prof_enable(self, on=True):
+        self._check("heipa_prof_enable", self.L.heipa_prof_enable(self.h, int(on)))
+
+    def prof_reset(self):
+        self._check("heipa_prof_reset", self.L.heipa_prof_reset(self.h))
+
+    def prof_stats(self):
+        out = {}
+        cnt = self.L.heipa_prof_count(self.h)
+        buf = ctypes.create_string_buffer(128)
+        for i in range(cnt):
+            ln, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+            self._check("heipa_prof_get", self.L.heipa_prof_get(self.h, i, buf, 128, ctypes.byref(ln), ctypes.byref(ms),
+                                                                ctypes.byref(by)))
+            out[buf.value.decode()] = {"launches": ln.value, "ms": ms.value, "bytes": by.value}
+        return out
+
+
+def _download_graph(desc: Graph) -> HostGraph:
+    """Copies a device heipa_graph into a HostGraph via the HIP runtime of torch."""
+    torch = _torch()
+    n, m = desc.n, desc.m
+
+    def dl(ptr, count, dtype):
+        if count == 0 or not ptr:
+            return np.zeros(count, dtype)
+        buf = np.empty(count, dtype)
+        # hipMemcpy through torch: wrap the device pointer with a tensor view is not
+        # possible without ownership, so use the HIP runtime directly.
+        _hip_memcpy_d2h(buf.ctypes.data, ptr, buf.nbytes)
+        return buf
+
+    torch.cuda.synchronize()
+    xadj = dl(desc.xadj, n + 1, np.uint32)
+    adjncy = dl(desc.adjncy, m, np.uint32)
+    adjwgt = dl(desc.adjwgt, m, np.int32) if desc.adjwgt else None
+    vwgt = dl(desc.vwgt, n, np.int32) if desc.vwgt else None
+    return HostGraph(xadj, adjncy, adjwgt, vwgt)
+
+
+_hip = None
+
+
+def _hip_memcpy_d2h(dst, src, nbytes):
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = ctypes.c_int
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rc = _hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, 2)  # hipMemcpyDeviceToHost
+    if rc != 0:
+        raise HeipaError("hipMemcpy", ERR_DEVICE)
+
+
+def to_numpy_u32(t):
+    """torch int32 tensor holding u32 values -> numpy uint32."""
+    return t.cpu().numpy().view(np.uint32)
+
+
+def from_numpy_u32(a, device="cuda:0"):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint32).view(np.int32)).to(device)

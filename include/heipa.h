@@ -1,0 +1,180 @@
+/* heipa.h — C-ABI of the MI355X-native GPU-HeiPa / GPU-HeiProMap hot path.
+ *
+ * Plain pointers and sizes only.  Graph and partition arrays passed to the
+ * device entry points are DEVICE pointers (hipMalloc'd or from any allocator
+ * on the context's device); results come back as host scalars or in device
+ * arrays the caller provides.  Every call is synchronous at return (the
+ * reference fences at the end of every phase, solver.h:440/467/519/548) and
+ * returns a status code instead of calling exit()/abort().
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef HEIPA_H
+#define HEIPA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference aborts or exits instead:
+ * kokkos_memory_stack.h:83-94, host_graph.h:148-151, configuration.h:141-164) */
+#define HEIPA_OK 0
+#define HEIPA_ERR_OOM 1       /* context arena exhausted */
+#define HEIPA_ERR_INVALID 2   /* bad argument or malformed graph/file */
+#define HEIPA_ERR_OVERFLOW 3  /* a coarse weight does not fit the s32 graph format */
+#define HEIPA_ERR_DEVICE 4    /* HIP runtime error */
+#define HEIPA_ERR_IO 5        /* file error */
+
+/* heipa_graph.flags (the reference's uniform_vertex_weights / uniform_edge_weights,
+ * src/datastructures/graph.h:51-52: uniform weights are not stored) */
+#define HEIPA_UNIFORM_VW 1u
+#define HEIPA_UNIFORM_EW 2u
+
+typedef struct heipa_ctx heipa_ctx;
+
+/* Device CSR graph; replaces `Graph` (src/datastructures/graph.h:47-61).
+ * Both directions stored, m = directed entries, u32 ids and offsets, s32
+ * weights.  The reference's redundant COO array edges_u is not kept. */
+typedef struct {
+    uint32_t n;
+    uint32_t m;
+    int64_t g_weight;          /* total vertex weight */
+    uint32_t flags;            /* HEIPA_UNIFORM_VW | HEIPA_UNIFORM_EW */
+    uint32_t reserved;
+    const uint32_t *xadj;      /* device, n + 1 */
+    const uint32_t *adjncy;    /* device, m */
+    const int32_t *adjwgt;     /* device, m, NULL iff HEIPA_UNIFORM_EW */
+    const int32_t *vwgt;       /* device, n, NULL iff HEIPA_UNIFORM_VW */
+} heipa_graph;
+
+typedef struct {
+    uint32_t rounds;
+    uint32_t after_hem, after_leaf, after_twin, after_relative;
+    uint32_t ran_leaf, ran_twin, ran_relative;
+} heipa_match_stats;
+
+/* one record per performed Jet iteration (see heipa_jet_refine_traced) */
+typedef struct {
+    uint32_t ratio_idx, iteration, mode, n_cand, n_moves, pad;
+    int64_t cut, maxw;
+    uint64_t moves_hash, gain_hash;
+} heipa_jet_trace;
+
+/* one record per level of a Solver run, finest first */
+typedef struct {
+    uint32_t n, m, nc, pad;
+    uint64_t mapping_hash;
+    int64_t cut, maxw;
+} heipa_level_trace;
+
+typedef struct {
+    int64_t cut;               /* final edge cut (64-bit) */
+    int64_t maxw;              /* final max block weight */
+    int64_t lmax;              /* ceil((1+eps) W / k) */
+    int64_t initial_cut;       /* cut of the initial partition on the coarsest graph */
+    uint32_t levels;           /* number of graphs in the hierarchy */
+    uint32_t jet_iterations;   /* Jet iterations over all levels */
+    double ms_coarsening, ms_contraction, ms_initial, ms_uncontraction, ms_refinement;
+} heipa_solve_stats;
+
+/* ---- context: device, stream, LIFO arena (replaces KokkosMemoryStack,
+ * src/datastructures/kokkos_memory_stack.h:36-187, and the default execution
+ * space).  One context per host thread. */
+int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out);
+int heipa_ctx_destroy(heipa_ctx *ctx);
+const char *heipa_ctx_last_error(const heipa_ctx *ctx);
+void *heipa_ctx_stream(heipa_ctx *ctx);          /* the hipStream_t all work runs on */
+uint64_t heipa_arena_mark(const heipa_ctx *ctx); /* front-stack depth, for heipa_arena_release */
+int heipa_arena_release(heipa_ctx *ctx, uint64_t mark);
+uint64_t heipa_arena_peak(const heipa_ctx *ctx); /* peak bytes in use */
+
+/* ---- evaluators */
+/* edge_cut<uew>(const Graph&, const Partition&, exec) — src/utility/edge_cut.h:71-90 (64-bit sum, halved) */
+int heipa_edge_cut(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, int64_t *cut);
+/* comm_cost<uew,O>(const Graph&, const Partition&, O&, exec) — src/utility/comm_cost.h:36-56
+ * (64-bit, not halved); D is a device k*k s32 matrix */
+int heipa_comm_cost(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, const int32_t *D, uint32_t k,
+                    int64_t *J);
+/* initialize_distance_oracle_matrix(k, hierarchy, distance, ...) —
+ * src/distance_oracles/distance_oracle_matrix.h:41-90; host output, k = prod(hier) */
+int heipa_distance_matrix(const uint32_t *hier, const int32_t *dist, uint32_t l, int32_t *D_host);
+/* recalculate_weights + max_weight — src/datastructures/partition.h:107-134; bw_host has k entries */
+int heipa_block_weights(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, uint32_t k, int64_t *bw_host,
+                        int64_t *maxw);
+
+/* ---- coarsening */
+/* two_hop_matcher_get_mapping<uvw,uew>(const Graph&, const Partition&, lmax, stack, exec) —
+ * src/coarsening/two_hop_matching.h:573-650.  mapping: device, n entries; seed 12345 in the Solver. */
+int heipa_match(heipa_ctx *ctx, const heipa_graph *g, uint32_t seed, uint32_t *mapping, uint32_t *nc,
+                heipa_match_stats *stats);
+/* from_Graph_Mapping<uvw,uew>(const Graph&, const Mapping&, stack, exec) — src/datastructures/graph.h:127-390.
+ * The coarse arrays are allocated on the context arena's front stack (LIFO, like the reference); free
+ * them with heipa_arena_release(ctx, mark taken before the call).  Rows are sorted by neighbour id. */
+int heipa_contract(heipa_ctx *ctx, const heipa_graph *fine, const uint32_t *mapping, uint32_t nc,
+                   heipa_graph *coarse);
+/* uncontract(Partition&, const Mapping&, exec) — src/datastructures/partition.h:92-105 */
+int heipa_project(heipa_ctx *ctx, const uint32_t *mapping, uint32_t fine_n, const uint32_t *coarse_part,
+                  uint32_t *fine_part);
+
+/* ---- refinement */
+/* jet_refine<uvw,uew>(Graph&, Partition&, k, lmax, use_ultra, level, cut, maxw, stack, exec) —
+ * src/refinement/jet_label_propagation.h:926-1033.  part is in/out (device); cut and maxw are in/out
+ * (the current values on entry, the best partition's on return). */
+int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
+                     uint32_t level, int64_t *cut, int64_t *maxw);
+/* same, recording one heipa_jet_trace per iteration (test hook; adds a readback per iteration) */
+int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax,
+                            int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
+                            uint32_t trace_cap, uint32_t *trace_len);
+
+/* ---- drivers */
+/* Solver(config).solve(host_g) / the device-graph Solver ctor — src/datastructures/solver.h:144-246, 248,
+ * 340-556.  part: device, n entries.  stats and levels may be NULL. */
+int heipa_partition(heipa_ctx *ctx, const heipa_graph *g, uint32_t k, double imbalance, uint64_t seed, int ultra,
+                    uint32_t *part, heipa_solve_stats *stats, heipa_level_trace *levels, uint32_t level_cap);
+/* ProMapSolver<DistanceOracleMatrix>(config).solve_multisection(host_g) —
+ * src/datastructures/promap_solver.h:238-241 -> src/initial_partitioning/hierarchical_multisection.h:237-307 */
+int heipa_hm_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hier, uint32_t l, double imbalance,
+                       uint64_t seed, int ultra, uint32_t *part);
+/* determine_adaptive_imbalance — src/initial_partitioning/global_multisection.h:39-48 */
+double heipa_adaptive_imbalance(double global_imbalance, int64_t global_g_weight, uint32_t global_k,
+                                int64_t local_g_weight, uint32_t local_k_rem, uint64_t depth);
+
+/* ---- per-kernel timing (HIP events on the context stream) */
+int heipa_prof_enable(heipa_ctx *ctx, int on);
+int heipa_prof_reset(heipa_ctx *ctx);
+uint32_t heipa_prof_count(heipa_ctx *ctx);
+/* algorithmic bytes = the compulsory HBM bytes of each launch, summed */
+int heipa_prof_get(heipa_ctx *ctx, uint32_t idx, char *name, uint32_t name_cap, uint64_t *launches,
+                   double *total_ms, double *alg_bytes);
+
+/* ---- host-side graph tools (no device use) */
+typedef struct heipa_host_graph heipa_host_graph;
+/* from_file — src/datastructures/host_graph.h:154-247 */
+int heipa_read_metis(const char *path, heipa_host_graph **out);
+int heipa_write_metis(const char *path, const heipa_host_graph *g);
+/* write_partition — src/utility/util.h:288-329 */
+int heipa_write_partition(const char *path, const uint32_t *part_host, uint32_t n);
+int heipa_gen_grid(uint32_t rows, uint32_t cols, heipa_host_graph **out);
+int heipa_gen_rgg(uint32_t n, double avg_deg, uint64_t seed, uint32_t wmod, heipa_host_graph **out);
+int heipa_gen_rmat(uint32_t scale, uint32_t n, double edge_factor, double a, double b, double c, uint64_t seed,
+                   uint32_t wmod, int explicit_vw, heipa_host_graph **out);
+int heipa_host_graph_from_arrays(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
+                                 const int32_t *vwgt, heipa_host_graph **out);
+/* host views into the graph (valid until heipa_host_graph_free); adjwgt / vwgt NULL when uniform */
+int heipa_host_graph_view(const heipa_host_graph *g, uint32_t *n, uint32_t *m, int64_t *g_weight, uint32_t *flags,
+                          const uint32_t **xadj, const uint32_t **adjncy, const int32_t **adjwgt,
+                          const int32_t **vwgt);
+void heipa_host_graph_free(heipa_host_graph *g);
+/* the shared host initial partitioner (replaces kway_partition, kway_core.h:1548-1588); host arrays */
+int heipa_initial_partition(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
+                            const int32_t *vwgt, uint32_t k, double imbalance, uint64_t seed, uint32_t *part);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEIPA_H */

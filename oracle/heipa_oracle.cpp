@@ -1,0 +1,1046 @@
+// CPU oracle for the GPU-HeiPa multilevel hot path — TEST INFRASTRUCTURE ONLY.
+// See heipa_oracle.h for the contract.  Deterministic ("det") restatement of
+// the reference; every race the reference resolves by atomic order is resolved
+// here by the rules below, which equal the reference's serial-execution order
+// wherever one exists:
+//   D1 matching commit (two_hop_matching.h:321-329): the smallest proposer id
+//      wins a contested receiver.
+//   D2 leaf/twin/relative pairing (two_hop_matching.h:205-253, :427-571):
+//      vertices with exactly equal keys are paired (1,2),(3,4),... in id order;
+//      isolated vertices form one group under their key (no slot aliasing).
+//   D3 contraction (graph.h:127-390): every coarse row is sorted by neighbour id.
+//   D4 block-connectivity ties (block_conn.h slot order, jet:219-221, :662-665):
+//      the smallest block id wins; only blocks with positive connection exist.
+//   D5 minibucket offsets (jet:439, :684): stable rank by vertex id inside the
+//      minibucket; strong-rebalance overflow (jet:580) resolved serially in
+//      vertex id order.
+//   D6 gain_bucket (jet:147-180): exact comparison of the float quotient with
+//      powers of 1.5 instead of float log2 with an epsilon.
+// All sums (cut, J, block weights, connections, gains) are 64-bit.
+#include "heipa_oracle.h"
+
+#include "../gpu-heipa_amd/csrc/host/host_graph.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+using heipa::s32;
+using heipa::s64;
+using heipa::u32;
+using heipa::u64;
+using heipa::u8;
+
+namespace {
+
+constexpr u32 SENT = 0xFFFFFFFFu;           // two_hop_matching.h:49
+constexpr u32 RECV = SENT - 1;              // receiver mark, two_hop_matching.h:317
+constexpr u32 NO_MOVE = 0xFFFFFFFFu - 3;    // definitions.h:54
+constexpr u32 NULL_PART = 0xFFFFFFFFu - 1;  // definitions.h:52
+constexpr s64 GAIN_MIN = std::numeric_limits<s64>::min();
+constexpr u32 N_MAX_ITERATIONS = 12;        // jet:36
+constexpr u32 N_MAX_WEAK_ITERATIONS = 2;    // jet:37
+constexpr double PHI = 0.999;               // jet:38
+constexpr u32 MAX_SECTIONS = 128;           // jet:41
+constexpr u32 MAX_BUCKETS = 50;             // jet:42
+constexpr int MID_BUCKETS = 25;             // jet:43
+
+struct G {
+    u32 n = 0, m = 0;
+    std::vector<u32> xadj, adj;
+    std::vector<s32> ew, vw;
+    bool uew = true, uvw = true;
+    s64 gw = 0;
+    s64 w(u32 j) const { return uew ? 1 : ew[j]; }
+    s64 v(u32 u) const { return uvw ? 1 : vw[u]; }
+    u32 deg(u32 u) const { return xadj[u + 1] - xadj[u]; }
+};
+
+G from_view(const orc_graph *g) {
+    G r;
+    r.n = g->n;
+    r.m = g->m;
+    r.xadj.assign(g->xadj, g->xadj + g->n + 1);
+    r.adj.assign(g->adjncy, g->adjncy + g->m);
+    if (g->adjwgt) {
+        r.uew = false;
+        r.ew.assign(g->adjwgt, g->adjwgt + g->m);
+    }
+    if (g->vwgt) {
+        r.uvw = false;
+        r.vw.assign(g->vwgt, g->vwgt + g->n);
+    }
+    r.gw = g->g_weight;
+    return r;
+}
+
+// ---------------------------------------------------------------- hashing
+u32 rotl32(u32 x, int r) { return (x << r) | (x >> (32 - r)); }
+u32 fmix32(u32 h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Published MurmurHash3_x86_32 (SMHasher); Kokkos::pod_hash<u32> applies it to
+// the 4 little-endian bytes of the key with seed 0 (two_hop_matching.h:42).
+u32 murmur3(const void *key, int len, u32 seed) {
+    const u8 *data = (const u8 *) key;
+    const int nblocks = len / 4;
+    u32 h1 = seed;
+    const u32 c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+    for (int i = 0; i < nblocks; ++i) {
+        u32 k1 = (u32) data[4 * i] | ((u32) data[4 * i + 1] << 8) | ((u32) data[4 * i + 2] << 16) |
+                 ((u32) data[4 * i + 3] << 24);
+        k1 *= c1;
+        k1 = rotl32(k1, 15);
+        k1 *= c2;
+        h1 ^= k1;
+        h1 = rotl32(h1, 13);
+        h1 = h1 * 5 + 0xe6546b64u;
+    }
+    const u8 *tail = data + nblocks * 4;
+    u32 k1 = 0;
+    switch (len & 3) {
+        case 3: k1 ^= (u32) tail[2] << 16; [[fallthrough]];
+        case 2: k1 ^= (u32) tail[1] << 8; [[fallthrough]];
+        case 1:
+            k1 ^= tail[0];
+            k1 *= c1;
+            k1 = rotl32(k1, 15);
+            k1 *= c2;
+            h1 ^= k1;
+    }
+    h1 ^= (u32) len;
+    return fmix32(h1);
+}
+
+u32 pod_hash(u32 x) {
+    u8 b[4] = {(u8) x, (u8) (x >> 8), (u8) (x >> 16), (u8) (x >> 24)};
+    return murmur3(b, 4, 0);
+}
+
+// two_hop_matching.h:93-100
+u32 xorshift(u32 x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+u64 mix64(u64 z) { return heipa::mix64(z); }
+
+// ---------------------------------------------------------------- gain bucket
+// jet:147-180.  g = (float)(-gx) / (float)vwgt exactly as the reference; then
+// the bucket offset is the least k with 1.5^k >= g (g >= 1) or 1.5^k * g >= 1
+// (g < 1), decided by exact integer arithmetic on g's binary representation.
+bool float_times_pow3_ge_pow2(float g, int k) {
+    // g * 3^k >= 2^k  <=>  M * 3^k >= 2^(k - E) where g = M * 2^E, M integer
+    int e;
+    double fr = std::frexp((double) g, &e);       // g = fr * 2^e, fr in [0.5,1)
+    u64 M = (u64) std::ldexp(fr, 24);             // exact: float mantissa has 24 bits
+    int E = e - 24;
+    __int128 lhs = (__int128) M;
+    for (int i = 0; i < k; ++i) lhs *= 3;
+    int sh = k - E;
+    if (sh <= 0) return true;                      // rhs <= 1 <= lhs
+    if (sh >= 126) return false;
+    __int128 rhs = (__int128) 1 << sh;
+    return lhs >= rhs;
+}
+
+u32 gain_bucket(s64 gx, s64 vwgt) {
+    if (gx > 0) return 0;
+    if (gx == 0) return 1;
+    const float g = (float) (-gx) / (float) vwgt;
+    if (g < 1.0f) {
+        for (int k = 1; k <= 23; ++k)
+            if (float_times_pow3_ge_pow2(g, k)) return (u32) (MID_BUCKETS - k);
+        return 2;
+    }
+    double p = 1.0;  // 1.5^k exact in double for k <= 33
+    for (int k = 0; k <= 24; ++k) {
+        if ((double) g <= p) return (u32) (MID_BUCKETS + k);
+        p *= 1.5;
+    }
+    return MAX_BUCKETS - 1;
+}
+
+// ---------------------------------------------------------------- evaluators
+s64 edge_cut(const G &g, const u32 *part) {
+    s64 s = 0;
+    for (u32 u = 0; u < g.n; ++u)
+        for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j)
+            if (part[u] != part[g.adj[j]]) s += g.w(j);
+    return s / 2;  // edge_cut.h:89
+}
+
+s64 comm_cost(const G &g, const u32 *part, const s32 *D, u32 k) {
+    s64 s = 0;
+    for (u32 u = 0; u < g.n; ++u)
+        for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j)
+            s += g.w(j) * (s64) D[(u64) part[u] * k + part[g.adj[j]]];
+    return s;  // comm_cost.h:55, not halved
+}
+
+// ---------------------------------------------------------------- matching
+// pick_neighbor_flat / pick_neighbor_team (two_hop_matching.h:102-188): the
+// result is the eligible neighbour maximising (edge weight, xorshift(v + r))
+// lexicographically (weights ignored when uniform); SENT if none.
+u32 pick(const G &g, const std::vector<u32> &vcmap, u32 u, u32 seed, bool initial) {
+    const u32 r = xorshift(u ^ seed);
+    u32 h = SENT;
+    s64 bw = 0;
+    u32 bt = 0;
+    for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+        const u32 v = g.adj[j];
+        if (!initial && vcmap[v] != SENT) continue;
+        const s64 w = g.uew ? 1 : g.ew[j];
+        const u32 t = xorshift(v + r);
+        if (h == SENT || w > bw || (w == bw && t > bt)) {
+            h = v;
+            bw = w;
+            bt = t;
+        }
+    }
+    return h;
+}
+
+// D2: pair equal keys consecutively by id.  `items` is (key, vertex) in id order.
+template<typename K>
+void pair_by_key(std::vector<std::pair<K, u32>> &items, std::vector<u32> &vcmap) {
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<K, u32> &a, const std::pair<K, u32> &b) { return a.first < b.first; });
+    size_t i = 0;
+    while (i < items.size()) {
+        size_t j = i;
+        while (j < items.size() && items[j].first == items[i].first) ++j;
+        for (size_t t = i; t + 1 < j; t += 2) {
+            const u32 a = items[t].second, b = items[t + 1].second;
+            const u32 cv = a < b ? a : b;
+            vcmap[a] = cv;
+            vcmap[b] = cv;
+        }
+        i = j;
+    }
+}
+
+u32 count_unmapped(const std::vector<u32> &vcmap) {
+    u32 c = 0;
+    for (u32 x: vcmap) c += x == SENT;
+    return c;
+}
+
+// two_hop_matcher_get_mapping (two_hop_matching.h:573-650)
+u32 match(const G &g, u32 seed, std::vector<u32> &mapping, orc_match_stats *st) {
+    const u32 n = g.n;
+    std::vector<u32> vcmap(n, SENT), hn(n, SENT);
+    orc_match_stats local{};
+    // heavy_edge_matching (two_hop_matching.h:271-393)
+    for (u32 u = 0; u < n; ++u) {
+        if (g.uew) {
+            const u32 d = g.deg(u);
+            if (d == 0) continue;
+            hn[u] = g.adj[g.xadj[u] + xorshift(u ^ seed) % d];  // :279-284
+        } else {
+            hn[u] = pick(g, vcmap, u, seed, true);
+        }
+    }
+    std::vector<u32> list(n);
+    std::iota(list.begin(), list.end(), 0u);
+    u32 round = 0;
+    while (!list.empty()) {
+        const u32 round_seed = seed ^ (round * 0x9e3779b1u);
+        for (u32 r = 0; r < 4; ++r) {
+            for (u32 u: list) {  // commit_p1 :313-320
+                const u32 v = hn[u];
+                if (v == SENT || vcmap[u] != SENT) continue;
+                const bool cond = r > 0 ? pod_hash(u + r) < pod_hash(v + r) : u < v;
+                if (!cond) vcmap[u] = RECV;
+            }
+            for (u32 u: list) {  // commit_p2 :321-329, D1 (list is in id order)
+                const u32 v = hn[u];
+                if (v == SENT || vcmap[u] != SENT) continue;
+                const u32 cv = u < v ? u : v;
+                if (vcmap[v] == RECV) {
+                    vcmap[v] = cv;
+                    vcmap[u] = cv;
+                }
+            }
+            for (u32 u: list)  // commit_p3 :330-333
+                if (vcmap[u] == RECV) vcmap[u] = SENT;
+        }
+        // repick :337-371 (is_initial = false)
+        for (u32 u: list) {
+            if (vcmap[u] != SENT || hn[u] == SENT || vcmap[hn[u]] == SENT) continue;
+            hn[u] = pick(g, vcmap, u, round_seed, false);
+        }
+        // compact :373-388
+        std::vector<u32> next;
+        for (u32 u: list)
+            if (vcmap[u] == SENT && hn[u] != SENT) next.push_back(u);
+        list.swap(next);
+        ++round;
+    }
+    local.rounds = round;
+    u32 unmapped = count_unmapped(vcmap);
+    local.after_hem = local.after_leaf = local.after_twin = local.after_relative = unmapped;
+    if ((double) unmapped / (double) n > 0.25) {  // leaf :395-425
+        local.ran_leaf = 1;
+        std::vector<std::pair<u32, u32>> items;
+        for (u32 u = 0; u < n; ++u)
+            if (vcmap[u] == SENT && g.deg(u) == 1) items.emplace_back(g.adj[g.xadj[u]], u);
+        pair_by_key(items, vcmap);
+        unmapped = count_unmapped(vcmap);
+    }
+    local.after_leaf = local.after_twin = local.after_relative = unmapped;
+    if ((double) unmapped / (double) n > 0.25) {  // twin :427-524
+        local.ran_twin = 1;
+        std::vector<std::pair<u64, u32>> items;
+        for (u32 u = 0; u < n; ++u) {
+            if (vcmap[u] != SENT) continue;
+            u64 h = 0;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                u64 y = pod_hash(g.adj[j]);
+                h += y * y + y;
+            }
+            items.emplace_back(h, u);
+        }
+        pair_by_key(items, vcmap);
+        unmapped = count_unmapped(vcmap);
+    }
+    local.after_twin = local.after_relative = unmapped;
+    if ((double) unmapped / (double) n > 0.25) {  // relative :526-571
+        local.ran_relative = 1;
+        std::vector<std::pair<u32, u32>> items;
+        for (u32 u = 0; u < n; ++u) {
+            if (vcmap[u] != SENT) continue;
+            u32 h = SENT;
+            s64 max_w = 0;
+            u32 min_deg = SENT;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                const u32 vd = g.deg(v);
+                const s64 w = g.uew ? 1 : g.ew[j];
+                if (min_deg > vd) {
+                    min_deg = vd;
+                    max_w = w;
+                    h = v;
+                } else if (min_deg == vd && max_w < w) {
+                    h = v;
+                    max_w = w;
+                }
+            }
+            items.emplace_back(h, u);
+        }
+        pair_by_key(items, vcmap);
+        local.after_relative = count_unmapped(vcmap);
+    }
+    // build_mapping :613-641
+    for (u32 u = 0; u < n; ++u)
+        if (vcmap[u] == SENT) vcmap[u] = u;
+    std::vector<u32> cid(n, SENT);
+    u32 nc = 0;
+    for (u32 u = 0; u < n; ++u)
+        if (vcmap[u] == u) cid[u] = nc++;
+    mapping.resize(n);
+    for (u32 u = 0; u < n; ++u) mapping[u] = cid[vcmap[u]];
+    if (st) *st = local;
+    return nc;
+}
+
+// ---------------------------------------------------------------- contraction
+// from_Graph_Mapping (graph.h:127-390), rows sorted (D3).  Returns false if a
+// coarse edge weight does not fit in s32.
+bool contract(const G &g, const std::vector<u32> &map, u32 nc, G &c) {
+    c = G{};
+    c.n = nc;
+    c.uew = false;
+    c.uvw = false;
+    c.gw = g.gw;
+    c.vw.assign(nc, 0);
+    std::vector<s64> cvw(nc, 0);
+    std::vector<u32> cnt(nc + 1, 0), members(g.n);
+    for (u32 u = 0; u < g.n; ++u) {
+        cvw[map[u]] += g.v(u);
+        cnt[map[u] + 1]++;
+    }
+    for (u32 i = 0; i < nc; ++i) cnt[i + 1] += cnt[i];
+    {
+        std::vector<u32> pos(cnt.begin(), cnt.end() - 1);
+        for (u32 u = 0; u < g.n; ++u) members[pos[map[u]]++] = u;
+    }
+    bool ok = true;
+    for (u32 i = 0; i < nc; ++i) {
+        if (cvw[i] > INT32_MAX) ok = false;
+        c.vw[i] = (s32) cvw[i];
+    }
+    std::vector<s64> acc(nc, 0);
+    std::vector<u32> touched;
+    c.xadj.assign(nc + 1, 0);
+    for (u32 cv = 0; cv < nc; ++cv) {
+        touched.clear();
+        for (u32 t = cnt[cv]; t < cnt[cv + 1]; ++t) {
+            const u32 u = members[t];
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 x = map[g.adj[j]];
+                if (x == cv) continue;
+                if (acc[x] == 0) touched.push_back(x);
+                acc[x] += g.w(j);
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (u32 x: touched) {
+            c.adj.push_back(x);
+            if (acc[x] > INT32_MAX) ok = false;
+            c.ew.push_back((s32) acc[x]);
+            acc[x] = 0;
+        }
+        c.xadj[cv + 1] = (u32) c.adj.size();
+    }
+    c.m = (u32) c.adj.size();
+    return ok;
+}
+
+// ---------------------------------------------------------------- Jet
+struct Conn {
+    std::vector<s64> acc;
+    std::vector<u32> touched;
+    explicit Conn(u32 k) : acc(k, 0) {}
+    void compute(const G &g, const std::vector<u32> &part, u32 u) {
+        touched.clear();
+        for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+            const u32 b = part[g.adj[j]];
+            if (acc[b] == 0) touched.push_back(b);
+            acc[b] += g.w(j);
+        }
+    }
+    void reset() {
+        for (u32 b: touched) acc[b] = 0;
+        touched.clear();
+    }
+};
+
+struct Jet {
+    const G &g;
+    u32 k;
+    s64 lmax;
+    std::vector<u32> part;  // working partition (lp.partition.map)
+    std::vector<s64> bw;    // lp.partition.bweights
+    std::vector<u32> dest_part, dest_cache;
+    std::vector<s64> gain_cache, gain1;
+    std::vector<u8> lock;
+    Conn conn;
+    s64 cut = 0, maxw = 0;
+
+    Jet(const G &gg, u32 kk, s64 lm) : g(gg), k(kk), lmax(lm), conn(kk) {}
+
+    void compute_sections(u32 &sections, u32 &t, u32 &width) const {  // jet:364-371, :605-612
+        u32 s = MAX_SECTIONS;
+        u32 section_size = (g.n + s * k) / (s * k);
+        if (section_size < 4096) {
+            section_size = 4096;
+            s = (g.n + section_size * k) / (section_size * k);
+        }
+        sections = s;
+        t = MAX_BUCKETS * k * s;
+        width = MAX_BUCKETS * s;
+    }
+
+    // jet_lp (jet:182-354); returns the move list
+    std::vector<u32> lp(double ratio, u32 &n_cand, u64 &gain_hash) {
+        std::vector<u32> cand;
+        for (u32 u = 0; u < g.n; ++u) {  // best_block :193-238
+            if (dest_cache[u] != NULL_PART) {
+                dest_part[u] = dest_cache[u];
+                continue;
+            }
+            const u32 u_id = part[u];
+            conn.compute(g, part, u);
+            const s64 own = conn.acc[u_id];
+            u32 best_id = NO_MOVE;
+            s64 best_conn = 0;
+            for (u32 b: conn.touched) {  // D4
+                if (b == u_id) continue;
+                const s64 c = conn.acc[b];
+                if (c > best_conn || (c == best_conn && best_id != NO_MOVE && b < best_id)) {
+                    best_conn = c;
+                    best_id = b;
+                }
+            }
+            conn.reset();
+            s64 gain = 0;
+            if (best_id != NO_MOVE) {
+                if (best_conn >= own || ((double) own - (double) best_conn) < std::floor(ratio * (double) own))
+                    gain = best_conn - own;
+                else
+                    best_id = NO_MOVE;
+            }
+            gain_cache[u] = gain;
+            dest_cache[u] = best_id;
+            dest_part[u] = best_id;
+        }
+        gain_hash = 0;
+        for (u32 u = 0; u < g.n; ++u) {  // filter :247-258
+            if (dest_part[u] != NO_MOVE && lock[u] == 0) {
+                cand.push_back(u);
+                gain1[u] = gain_cache[u];
+                gain_hash += mix64(mix64(u) + (u64) gain1[u]);
+            } else {
+                gain1[u] = GAIN_MIN;
+                lock[u] = 0;
+            }
+        }
+        n_cand = (u32) cand.size();
+        for (u32 u: cand) {  // afterburner :269-328
+            const s64 ug = gain1[u];
+            const u32 o = part[u], nn = dest_part[u];
+            s64 change = 0;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                const s64 vg = gain1[v];
+                const bool first = vg > ug || (vg == ug && v < u);
+                if (!first) continue;
+                const u32 vn = dest_part[v], vo = part[v];
+                change += g.w(j) * ((s64) (vn == nn) - (s64) (vn == o) + (s64) (vo == o) - (s64) (vo == nn));
+            }
+            if (ug + change >= 0) lock[u] = 1;
+        }
+        std::vector<u32> moves;  // filter :337-348
+        for (u32 u: cand)
+            if (lock[u]) moves.push_back(u);
+        return moves;
+    }
+
+    // D5 helper: exclusive in-bucket offsets by vertex id, then the bucket prefix P
+    void bucket_offsets(const std::vector<u32> &cands, const std::vector<u32> &gid, u32 t,
+                        std::vector<s64> &P, std::vector<s64> &off) {
+        P.assign((size_t) t + 1, 0);
+        std::vector<s64> run((size_t) t + 1, 0);
+        off.assign(g.n, 0);
+        for (u32 u: cands) {  // cands are in id order
+            off[u] = run[gid[u]];
+            run[gid[u]] += g.v(u);
+        }
+        s64 acc = 0;
+        for (size_t i = 0; i <= t; ++i) {
+            P[i] = acc;
+            acc += run[i];
+        }
+    }
+
+    // rebalance_weak (jet:592-751)
+    std::vector<u32> weak(u32 &n_cand) {
+        const s64 opt = (g.gw + (s64) (k - 1)) / (s64) k;
+        s64 max_b_w = (s64) ((double) lmax * 0.99);
+        if (max_b_w < lmax - 100) max_b_w = lmax - 100;
+        u32 sections, t, width;
+        compute_sections(sections, t, width);
+        std::vector<u32> under;
+        for (u32 b = 0; b < k; ++b)
+            if (bw[b] < max_b_w) under.push_back(b);
+        std::vector<u32> gid(g.n, 0), cands;
+        for (u32 u = 0; u < g.n; ++u) {  // best_block :658-694
+            const u32 u_id = part[u];
+            u32 best = u_id;
+            s64 gain = 0;
+            const s64 wu = g.v(u);
+            if (bw[u_id] > lmax && (double) wu < 1.5 * (double) (bw[u_id] - opt)) {
+                conn.compute(g, part, u);
+                const s64 own = conn.acc[u_id];
+                s64 best_w = 0;
+                bool have = false;
+                for (u32 b: conn.touched) {  // D4
+                    if (bw[b] >= max_b_w) continue;
+                    const s64 c = conn.acc[b];
+                    if (c > best_w || (c == best_w && have && b < best)) {
+                        best = b;
+                        best_w = c;
+                        have = true;
+                    }
+                }
+                conn.reset();
+                gain = best_w - own;
+                if (best_w <= 0) {
+                    // the reference divides by the underloaded count here; with no
+                    // underloaded block (its UB case) the vertex stays
+                    best = under.empty() ? u_id : under[u % (u32) under.size()];
+                    gain = -own;
+                }
+            }
+            dest_part[u] = best;
+            if (u_id != best) {
+                const u32 bt = gain_bucket(gain, wu);
+                gid[u] = (MAX_BUCKETS * u_id + bt) * sections + (u % sections);
+                cands.push_back(u);
+            }
+        }
+        n_cand = (u32) cands.size();
+        std::vector<s64> P, off;
+        bucket_offsets(cands, gid, t, P, off);
+        std::vector<u32> moves;
+        for (u32 u: cands) {  // filter_scores :718-744
+            const u32 u_id = part[u];
+            const s64 tw = off[u] + (P[gid[u]] - P[(size_t) u_id * width]);
+            if (tw < bw[u_id] - lmax) moves.push_back(u);
+        }
+        return moves;
+    }
+
+    // rebalance_strong (jet:356-590)
+    std::vector<u32> strong(u32 &n_cand) {
+        const s64 opt = (g.gw + (s64) (k - 1)) / (s64) k;
+        const s64 max_b_w = std::max(opt + 1, (s64) ((double) lmax * 0.99));
+        u32 sections, t, width;
+        compute_sections(sections, t, width);
+        bool have_vw = false;
+        s64 max_vwgt = 0;  // find_max_vwgt :374-388
+        for (u32 b = 0; b < k; ++b) {
+            if (bw[b] < max_b_w) {
+                const s64 cap = max_b_w - bw[b];
+                if (!have_vw || cap > max_vwgt) max_vwgt = cap;
+                have_vw = true;
+            }
+        }
+        std::vector<u32> gid(g.n, 0), cands;
+        for (u32 u = 0; u < g.n; ++u) {  // score_candidates :399-435
+            const u32 u_id = part[u];
+            const s64 bwu = bw[u_id];
+            const s64 wu = g.v(u);
+            if (!(have_vw && bwu > lmax && wu <= 2 * max_vwgt && wu < 2 * (bwu - opt))) continue;
+            conn.compute(g, part, u);
+            const s64 own = conn.acc[u_id];
+            s64 count = 0, sum = 0;
+            for (u32 b: conn.touched) {
+                if (b == u_id) continue;
+                if (bw[b] < max_b_w) {
+                    sum += conn.acc[b];
+                    count += 1;
+                }
+            }
+            conn.reset();
+            if (count == 0) count = 1;
+            const s64 gain = sum / count - own;
+            const u32 bt = gain_bucket(gain, std::min(wu, bwu - lmax));
+            gid[u] = (MAX_BUCKETS * u_id + bt) * sections + (u % sections) + 1;  // +1 kept (jet:437)
+            cands.push_back(u);
+        }
+        n_cand = (u32) cands.size();
+        std::vector<s64> P, off;
+        bucket_offsets(cands, gid, t, P, off);
+        std::vector<s64> evict_adjust(k, 0), evict_start(k + 1, 0);
+        std::vector<u32> moves;
+        std::vector<s64> score(g.n, 0);
+        for (u32 u: cands) {  // filter_scores :472-500
+            const u32 u_id = part[u];
+            const s64 sc = off[u] + P[gid[u]] - P[(size_t) u_id * width];
+            const s64 limit = bw[u_id] - lmax;
+            if (sc < limit) {
+                if (sc + g.v(u) >= limit) evict_adjust[u_id] = sc + g.v(u);
+                moves.push_back(u);
+            }
+        }
+        s64 prefix = 0;  // cookie cutter :508-552
+        for (u32 p = 0; p < k; ++p) {
+            s64 add = evict_adjust[p];
+            const size_t begin = (size_t) MAX_BUCKETS * p * sections;
+            if (add == 0) add = P[begin + (size_t) MAX_BUCKETS * sections] - P[begin];
+            evict_adjust[p] = P[begin] - prefix;
+            prefix += add;
+        }
+        s64 counter = prefix;
+        s64 run = 0;
+        evict_start[0] = 0;
+        for (u32 p = 0; p < k; ++p) {
+            if (max_b_w > bw[p]) run += max_b_w - bw[p];
+            evict_start[p + 1] = run;
+        }
+        for (u32 u: moves) {  // adjust_scores :556-586, D5 serial overflow order
+            const u32 u_id = part[u];
+            s64 pos = off[u] + P[gid[u]] - evict_adjust[u_id];
+            const s64 wu = g.v(u);
+            s64 id = 0;
+            bool done = false;
+            while (id < (s64) k) {
+                while (id <= (s64) k && evict_start[id] <= pos) id++;
+                id--;
+                if (id < (s64) k && wu / 2 <= evict_start[id + 1] - pos) {
+                    dest_part[u] = (u32) id;
+                    done = true;
+                    break;
+                }
+                if (id < (s64) k) {
+                    pos = counter;
+                    counter += wu;
+                }
+            }
+            if (!done) dest_part[u] = part[u];
+        }
+        return moves;
+    }
+
+    // perform_moves (jet:753-924) with the exact 64-bit cut change
+    void apply(const std::vector<u32> &moves) {
+        std::vector<u8> moved(g.n, 0);
+        for (u32 u: moves) moved[u] = 1;
+        s64 delta = 0;  // cut increase
+        for (u32 u: moves) {
+            const u32 ou = part[u], nu = dest_part[u];
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                const u32 ov = part[v], nv = moved[v] ? dest_part[v] : part[v];
+                const s64 d = g.w(j) * ((s64) (nu != nv) - (s64) (ou != ov));
+                delta += moved[v] ? d : 2 * d;
+            }
+        }
+        cut += delta / 2;
+        for (u32 u: moves) {
+            const u32 ou = part[u], nu = dest_part[u];
+            bw[ou] -= g.v(u);
+            bw[nu] += g.v(u);
+            dest_cache[u] = NULL_PART;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) dest_cache[g.adj[j]] = NULL_PART;
+        }
+        for (u32 u: moves) part[u] = dest_part[u];
+        maxw = 0;
+        for (u32 b = 0; b < k; ++b) maxw = std::max(maxw, bw[b]);
+    }
+};
+
+u64 moves_hash(const std::vector<u32> &moves, const std::vector<u32> &dest) {
+    u64 h = 0;
+    for (u32 u: moves) h += mix64(((u64) u << 32) | dest[u]);
+    return h;
+}
+
+// jet_refine (jet:926-1033); part is in/out and receives the best partition.
+void jet_refine(const G &g, std::vector<u32> &part, u32 k, s64 lmax, bool ultra, u32 level, s64 &cut,
+                s64 &maxw, orc_jet_trace *trace, u32 trace_cap, u32 *trace_len) {
+    Jet J(g, k, lmax);
+    J.part = part;
+    J.bw.assign(k, 0);
+    for (u32 u = 0; u < g.n; ++u) J.bw[part[u]] += g.v(u);
+    J.dest_part.assign(g.n, 0);
+    J.dest_cache.assign(g.n, NULL_PART);
+    J.gain_cache.assign(g.n, 0);
+    J.gain1.assign(g.n, 0);
+    J.lock.assign(g.n, 0);
+    J.cut = cut;
+    J.maxw = maxw;
+    s64 best_cut = cut, best_maxw = maxw;
+    std::vector<double> ratios;
+    if (ultra) ratios = {0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45, 0.40, 0.35, 0.30, 0.25, 0.20, 0.15, 0.10, 0.05};
+    else ratios = {level == 0 ? 0.25 : 0.75};
+    u32 tl = 0;
+    for (u32 ri = 0; ri < ratios.size(); ++ri) {
+        u32 bal = 0, it = 0;
+        while (it < N_MAX_ITERATIONS) {
+            it += 1;
+            std::vector<u32> moves;
+            u32 mode, n_cand = 0;
+            u64 gh = 0;
+            if (J.maxw <= lmax) {
+                mode = 0;
+                moves = J.lp(ratios[ri], n_cand, gh);
+                bal = 0;
+                if (moves.empty()) break;
+            } else {
+                if (bal < N_MAX_WEAK_ITERATIONS) {
+                    mode = 1;
+                    moves = J.weak(n_cand);
+                    if (moves.empty()) {
+                        bal = N_MAX_WEAK_ITERATIONS;
+                        continue;
+                    }
+                } else {
+                    mode = 2;
+                    moves = J.strong(n_cand);
+                    if (moves.empty()) break;
+                }
+                bal++;
+            }
+            const u64 mh = moves_hash(moves, J.dest_part);
+            J.apply(moves);
+            if (trace && tl < trace_cap) {
+                orc_jet_trace &t = trace[tl];
+                t.ratio_idx = ri;
+                t.iteration = it;
+                t.mode = mode;
+                t.n_cand = n_cand;
+                t.n_moves = (u32) moves.size();
+                t.pad = 0;
+                t.cut = J.cut;
+                t.maxw = J.maxw;
+                t.moves_hash = mh;
+                t.gain_hash = gh;
+            }
+            ++tl;
+            if (best_maxw > lmax && J.maxw < best_maxw) {
+                part = J.part;
+                best_cut = J.cut;
+                best_maxw = J.maxw;
+                it = 0;
+            } else if (J.cut < best_cut && (J.maxw <= lmax || J.maxw <= best_maxw)) {
+                if ((double) J.cut < PHI * (double) best_cut) it = 0;
+                part = J.part;
+                best_cut = J.cut;
+                best_maxw = J.maxw;
+            }
+        }
+    }
+    if (trace_len) *trace_len = tl;
+    cut = best_cut;
+    maxw = best_maxw;
+}
+
+// ---------------------------------------------------------------- Solver
+struct SolveOut {
+    std::vector<u32> part;
+    s64 cut = 0, maxw = 0;
+};
+
+// Solver level loop (solver.h:144-246 device ctor / :340-400 internal_solve)
+bool solve(const G &g0, u32 k, double imbalance, u64 seed, bool ultra, SolveOut &out,
+           std::vector<orc_level_trace> *levels) {
+    const s64 lmax = (s64) std::ceil((1.0 + imbalance) * ((double) g0.gw / (double) k));  // solver.h:410
+    std::vector<G> graphs;
+    std::vector<std::vector<u32>> maps;
+    graphs.push_back(g0);
+    const u32 max_n = 8 * k;  // solver.h:343
+    while (graphs.back().n > max_n) {
+        std::vector<u32> mp;
+        u32 nc = match(graphs.back(), 12345u, mp, nullptr);  // two_hop_matching.h:590
+        G c;
+        if (!contract(graphs.back(), mp, nc, c)) return false;
+        maps.push_back(std::move(mp));
+        graphs.push_back(std::move(c));
+    }
+    const G &cg = graphs.back();
+    std::vector<u32> part(cg.n);
+    heipa::initial_partition(cg.n, cg.xadj.data(), cg.adj.data(), cg.uew ? nullptr : cg.ew.data(),
+                             cg.uvw ? nullptr : cg.vw.data(), k, imbalance, seed, part.data());
+    s64 cut = edge_cut(cg, part.data());
+    s64 maxw = 0;
+    {
+        std::vector<s64> bw(k, 0);
+        for (u32 u = 0; u < cg.n; ++u) bw[part[u]] += cg.v(u);
+        for (u32 b = 0; b < k; ++b) maxw = std::max(maxw, bw[b]);
+    }
+    if (levels) {
+        levels->assign(graphs.size(), orc_level_trace{});
+        for (size_t l = 0; l < graphs.size(); ++l) {
+            auto &L = (*levels)[l];
+            L.n = graphs[l].n;
+            L.m = graphs[l].m;
+            L.nc = l + 1 < graphs.size() ? graphs[l + 1].n : 0;
+            if (l < maps.size()) {
+                u64 h = 0;
+                for (u32 u = 0; u < graphs[l].n; ++u) h += mix64(((u64) u << 32) | maps[l][u]);
+                L.mapping_hash = h;
+            }
+        }
+        levels->back().cut = cut;
+        levels->back().maxw = maxw;
+    }
+    for (size_t l = maps.size(); l-- > 0;) {
+        std::vector<u32> fine(graphs[l].n);
+        for (u32 u = 0; u < graphs[l].n; ++u) fine[u] = part[maps[l][u]];  // uncontract partition.h:92-105
+        part.swap(fine);
+        jet_refine(graphs[l], part, k, lmax, ultra, (u32) l, cut, maxw, nullptr, 0, nullptr);
+        if (levels) {
+            (*levels)[l].cut = cut;
+            (*levels)[l].maxw = maxw;
+        }
+    }
+    out.part = std::move(part);
+    out.cut = cut;
+    out.maxw = maxw;
+    return true;
+}
+
+double adaptive_imbalance(double eps, s64 gW, u32 gk, s64 lW, u32 krem, u64 depth) {
+    // global_multisection.h:39-48
+    double li = (1.0 + eps) * ((double) ((u64) krem * (u64) gW) / (double) ((u64) gk * (u64) lW));
+    return std::pow(li, 1.0 / (double) depth) - 1.0;
+}
+
+// recursive_multisection_device (hierarchical_multisection.h:56-235)
+bool hm_rec(const G &g, const std::vector<u32> &n_to_o, const std::vector<u32> &hier, u32 level, double eps,
+            s64 gW, u32 gk, u64 seed, bool ultra, const std::vector<u32> &index_vec,
+            const std::vector<u32> &k_rem, std::vector<u32> &ident, std::vector<u32> &global) {
+    const u32 l = (u32) hier.size();
+    const u32 k = hier[level];
+    const double imb = adaptive_imbalance(eps, gW, gk, g.gw, k_rem[l - 1 - ident.size()], l - ident.size());
+    std::vector<u32> tmp(g.n, 0);
+    if (k != 1) {  // gpu_heipa_partition :38-54
+        SolveOut so;
+        if (!solve(g, k, imb, seed, ultra, so, nullptr)) return false;
+        tmp = so.part;
+    }
+    if (ident.size() == l - 1) {  // leaf :87-101
+        u32 offset = 0;
+        for (u32 i = 0; i < l - 1; ++i) offset += ident[i] * index_vec[index_vec.size() - 1 - i];
+        for (u32 u = 0; u < g.n; ++u) global[n_to_o[u]] = offset + tmp[u];
+        return true;
+    }
+    for (u32 id = 0; id < k; ++id) {  // children :104-230
+        G c;
+        std::vector<u32> loc(g.n, SENT), c_n_to_o;
+        for (u32 u = 0; u < g.n; ++u) {
+            if (tmp[u] != id) continue;
+            loc[u] = c.n++;
+            c_n_to_o.push_back(n_to_o[u]);
+        }
+        if (c.n == 0) continue;
+        c.uew = false;
+        c.uvw = false;
+        c.xadj.assign(c.n + 1, 0);
+        s64 sw = 0;
+        for (u32 u = 0; u < g.n; ++u) {
+            if (tmp[u] != id) continue;
+            c.vw.push_back((s32) g.v(u));
+            sw += g.v(u);
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                if (tmp[v] != id) continue;
+                c.adj.push_back(loc[v]);
+                c.ew.push_back((s32) g.w(j));
+            }
+            c.xadj[loc[u] + 1] = (u32) c.adj.size();
+        }
+        c.m = (u32) c.adj.size();
+        c.gw = sw;
+        ident.push_back(id);
+        if (!hm_rec(c, c_n_to_o, hier, level - 1, eps, gW, gk, seed, ultra, index_vec, k_rem, ident, global))
+            return false;
+        ident.pop_back();
+    }
+    return true;
+}
+
+} // namespace
+
+extern "C" {
+
+uint32_t orc_murmur3_x86_32(const void *key, int32_t len, uint32_t seed) { return murmur3(key, len, seed); }
+uint32_t orc_pod_hash_u32(uint32_t x) { return pod_hash(x); }
+uint32_t orc_xorshift(uint32_t x) { return xorshift(x); }
+uint64_t orc_mix64(uint64_t x) { return mix64(x); }
+uint32_t orc_gain_bucket(int64_t gx, int64_t vwgt) { return gain_bucket(gx, vwgt); }
+
+int64_t orc_edge_cut(const orc_graph *g, const uint32_t *part) { return edge_cut(from_view(g), part); }
+
+int64_t orc_comm_cost(const orc_graph *g, const uint32_t *part, const int32_t *D, uint32_t k) {
+    return comm_cost(from_view(g), part, D, k);
+}
+
+int orc_distance_matrix(const uint32_t *hier, const int32_t *dist, uint32_t l, int32_t *D_out) {
+    u64 k = 1;
+    for (u32 i = 0; i < l; ++i) k *= hier[i];
+    for (u64 idx = 0; idx < k * k; ++idx) {
+        const u64 i = idx / k, j = idx % k;
+        if (i == j) {
+            D_out[idx] = 0;
+            continue;
+        }
+        u32 level = 0;
+        u64 group = 1;
+        for (; level < l; ++level) {
+            group *= hier[level];
+            if (i / group == j / group) break;
+        }
+        D_out[idx] = dist[level < l ? level : l - 1];
+    }
+    return 0;
+}
+
+int orc_match(const orc_graph *g, uint32_t seed, uint32_t *mapping, uint32_t *nc, orc_match_stats *stats) {
+    G gg = from_view(g);
+    std::vector<u32> mp;
+    *nc = match(gg, seed, mp, stats);
+    std::memcpy(mapping, mp.data(), sizeof(u32) * gg.n);
+    return 0;
+}
+
+int orc_contract(const orc_graph *g, const uint32_t *mapping, uint32_t nc, uint32_t *cxadj, uint32_t *cadjncy,
+                 int32_t *cadjwgt, int32_t *cvwgt, uint32_t *cm) {
+    G gg = from_view(g);
+    std::vector<u32> mp(mapping, mapping + gg.n);
+    G c;
+    const bool ok = contract(gg, mp, nc, c);
+    std::memcpy(cxadj, c.xadj.data(), sizeof(u32) * (nc + 1));
+    std::memcpy(cadjncy, c.adj.data(), sizeof(u32) * c.m);
+    std::memcpy(cadjwgt, c.ew.data(), sizeof(s32) * c.m);
+    std::memcpy(cvwgt, c.vw.data(), sizeof(s32) * nc);
+    *cm = c.m;
+    return ok ? 0 : 3;
+}
+
+void orc_project(const uint32_t *mapping, uint32_t fine_n, const uint32_t *coarse_part, uint32_t *fine_part) {
+    for (u32 u = 0; u < fine_n; ++u) fine_part[u] = coarse_part[mapping[u]];
+}
+
+int orc_jet_refine(const orc_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra, uint32_t level,
+                   int64_t *cut, int64_t *maxw, orc_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len) {
+    G gg = from_view(g);
+    std::vector<u32> p(part, part + gg.n);
+    s64 c = *cut, mw = *maxw;
+    jet_refine(gg, p, k, lmax, ultra != 0, level, c, mw, trace, trace_cap, trace_len);
+    std::memcpy(part, p.data(), sizeof(u32) * gg.n);
+    *cut = c;
+    *maxw = mw;
+    return 0;
+}
+
+int orc_partition(const orc_graph *g, uint32_t k, double imbalance, uint64_t seed, int ultra, uint32_t *part,
+                  int64_t *cut, int64_t *maxw, orc_level_trace *levels, uint32_t level_cap, uint32_t *n_levels) {
+    G gg = from_view(g);
+    SolveOut so;
+    std::vector<orc_level_trace> lv;
+    if (k == 0) return 2;
+    if (!solve(gg, k, imbalance, seed, ultra != 0, so, levels ? &lv : nullptr)) return 3;
+    std::memcpy(part, so.part.data(), sizeof(u32) * gg.n);
+    if (cut) *cut = so.cut;
+    if (maxw) *maxw = so.maxw;
+    if (levels) {
+        for (u32 i = 0; i < lv.size() && i < level_cap; ++i) levels[i] = lv[i];
+        if (n_levels) *n_levels = (u32) lv.size();
+    }
+    return 0;
+}
+
+double orc_adaptive_imbalance(double global_imbalance, int64_t global_g_weight, uint32_t global_k,
+                              int64_t local_g_weight, uint32_t local_k_rem, uint64_t depth) {
+    return adaptive_imbalance(global_imbalance, global_g_weight, global_k, local_g_weight, local_k_rem, depth);
+}
+
+int orc_hm_partition(const orc_graph *g, const uint32_t *hier, uint32_t l, double imbalance, uint64_t seed,
+                     int ultra, uint32_t *part) {
+    G gg = from_view(g);
+    std::vector<u32> h(hier, hier + l);
+    u32 gk = 1;
+    for (u32 x: h) gk *= x;
+    std::vector<u32> index_vec = {1};  // hierarchical_multisection.h:262-263
+    for (u32 i = 0; i + 1 < l; ++i) index_vec.push_back(index_vec[i] * h[i]);
+    std::vector<u32> k_rem(l);  // :265-270
+    u32 p = 1;
+    for (u32 i = 0; i < l; ++i) {
+        k_rem[i] = p * h[i];
+        p *= h[i];
+    }
+    std::vector<u32> n_to_o(gg.n);
+    std::iota(n_to_o.begin(), n_to_o.end(), 0u);
+    std::vector<u32> global(gg.n, 0), ident;
+    if (!hm_rec(gg, n_to_o, h, l - 1, imbalance, gg.gw, gk, seed, ultra != 0, index_vec, k_rem, ident, global))
+        return 3;
+    std::memcpy(part, global.data(), sizeof(u32) * gg.n);
+    return 0;
+}
+
+} // extern "C"

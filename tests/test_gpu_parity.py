@@ -1,0 +1,138 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle, bit-exact
+(integer work), through the C-ABI on cuda:0."""
+import math
+
+import numpy as np
+import pytest
+
+import heipa
+import heipa_oracle as orc
+from graphs import block_weights, random_partition, star_plus_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _graphs():
+    return {
+        "grid64": heipa.gen_grid(64, 64),
+        "grid_rect": heipa.gen_grid(37, 91),
+        "rgg14w": heipa.gen_rgg(1 << 14, 12.0, 1, 10),
+        "rgg12u": heipa.gen_rgg(1 << 12, 8.0, 5, 0),
+        "rmat12w": heipa.gen_rmat(12, 0, 16.0, 0.57, 0.19, 0.19, 2, 100, True),
+        "rmat11u": heipa.gen_rmat(11, 0, 8.0, 0.57, 0.19, 0.19, 3, 0, False),
+        "star": star_plus_grid(),
+    }
+
+
+GRAPHS = _graphs()
+
+
+@pytest.fixture(params=sorted(GRAPHS))
+def graph(request):
+    return request.param, GRAPHS[request.param]
+
+
+def _dev(g):
+    return heipa.DeviceGraph(g, "cuda:0")
+
+
+def test_edge_cut_comm_cost(ctx, graph):
+    name, g = graph
+    dg = _dev(g)
+    for k, seed in ((2, 1), (7, 2), (48, 3)):
+        part = random_partition(g.n, k, seed)
+        pd = heipa.from_numpy_u32(part)
+        assert ctx.edge_cut(dg, pd) == orc.edge_cut(g, part), (name, k)
+    hier, dist = [4, 8, 6], [1, 10, 100]
+    D = orc.distance_matrix(hier, dist)
+    part = random_partition(g.n, 192, 9)
+    Dd = heipa.from_numpy_u32(D.reshape(-1).view(np.uint32))
+    assert ctx.comm_cost(dg, heipa.from_numpy_u32(part), Dd, 192) == orc.comm_cost(g, part, D), name
+    bw, mx = ctx.block_weights(dg, heipa.from_numpy_u32(part), 192)
+    assert np.array_equal(bw, block_weights(g, part, 192))
+
+
+def test_match(ctx, graph):
+    name, g = graph
+    dg = _dev(g)
+    mp, nc, st = ctx.match(dg, 12345)
+    omp, onc, ost = orc.match(g, 12345)
+    assert nc == onc, (name, nc, onc)
+    assert np.array_equal(heipa.to_numpy_u32(mp), omp), name
+    for f in ("rounds", "after_hem", "after_leaf", "after_twin", "after_relative", "ran_leaf", "ran_twin",
+              "ran_relative"):
+        assert getattr(st, f) == getattr(ost, f), (name, f, getattr(st, f), getattr(ost, f))
+
+
+def test_contract_project(ctx, graph):
+    name, g = graph
+    dg = _dev(g)
+    omp, onc, _ = orc.match(g, 12345)
+    cg = ctx.contract(dg, heipa.from_numpy_u32(omp), onc)
+    cx, ca, cw, cv, rc = orc.contract(g, omp, onc)
+    assert rc == 0
+    assert np.array_equal(cg.xadj, cx), name
+    assert np.array_equal(cg.adjncy, ca), name
+    assert np.array_equal(cg.adjwgt, cw), name
+    assert np.array_equal(cg.vwgt, cv), name
+    cpart = random_partition(onc, 5, 4)
+    fine = ctx.project(heipa.from_numpy_u32(omp), heipa.from_numpy_u32(cpart), g.n)
+    assert np.array_equal(heipa.to_numpy_u32(fine), orc.project(omp, cpart))
+
+
+def _trace_tuple(t):
+    return (t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
+
+
+@pytest.mark.parametrize("ultra,level,skew", [(False, 0, 0.0), (False, 1, 0.0), (True, 0, 0.0), (False, 0, 0.6),
+                                              (True, 2, 0.9)])
+def test_jet(ctx, graph, ultra, level, skew):
+    name, g = graph
+    k = 8
+    dg = _dev(g)
+    # start from a coarse-ish partition: blocks of consecutive ids, perturbed
+    part0 = (np.arange(g.n, dtype=np.int64) * k // max(g.n, 1)).astype(np.uint32)
+    rnd = random_partition(g.n, k, 11, skew)
+    flip = np.random.default_rng(5).random(g.n) < 0.15
+    part0 = np.where(flip, rnd, part0).astype(np.uint32)
+    vw = np.ones(g.n, np.int64) if g.vwgt is None else g.vwgt.astype(np.int64)
+    lmax = math.ceil(1.03 * vw.sum() / k)
+    cut0 = orc.edge_cut(g, part0)
+    maxw0 = int(block_weights(g, part0, k).max())
+    op, ocut, omaxw, otr = orc.jet_refine(g, part0, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
+    pd = heipa.from_numpy_u32(part0)
+    gcut, gmaxw, gtr = ctx.jet_refine(dg, pd, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
+    assert len(gtr) == len(otr), (name, len(gtr), len(otr))
+    for i, (a, b) in enumerate(zip(gtr, otr)):
+        assert _trace_tuple(a) == _trace_tuple(b), (name, i, _trace_tuple(a), _trace_tuple(b))
+    assert (gcut, gmaxw) == (ocut, omaxw)
+    gp = heipa.to_numpy_u32(pd)
+    assert np.array_equal(gp, op), name
+    assert orc.edge_cut(g, gp) == gcut
+
+
+@pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False)])
+def test_partition(ctx, graph, k, ultra):
+    name, g = graph
+    dg = _dev(g)
+    part, st, lv = ctx.partition(dg, k, 0.03, 1, ultra, level_cap=64)
+    op, ocut, omaxw, olv = orc.partition(g, k, 0.03, 1, ultra)
+    assert len(lv) == len(olv), name
+    for a, b in zip(lv, olv):
+        assert (a.n, a.m, a.nc, a.mapping_hash, a.cut, a.maxw) == (b.n, b.m, b.nc, b.mapping_hash, b.cut, b.maxw), name
+    assert np.array_equal(heipa.to_numpy_u32(part), op), name
+    assert (st.cut, st.maxw) == (ocut, omaxw)
+    assert st.lmax == math.ceil((1.03) * g.g_weight / k)
+
+
+@pytest.mark.parametrize("hier", [[4, 8, 6], [2, 4, 8], [3, 1, 2]])
+def test_hm(ctx, hier):
+    g = heipa.gen_rmat(12, 0, 16.0, 0.57, 0.19, 0.19, 7, 100, True)
+    dg = _dev(g)
+    gp = heipa.to_numpy_u32(ctx.hm_partition(dg, hier, 0.03, 1, False))
+    op = orc.hm_partition(g, hier, 0.03, 1, False)
+    assert np.array_equal(gp, op)
+    k = int(np.prod(hier))
+    D = orc.distance_matrix(hier, [1, 10, 100])
+    Dd = heipa.from_numpy_u32(D.reshape(-1).view(np.uint32))
+    assert ctx.comm_cost(dg, heipa.from_numpy_u32(gp), Dd, k) == orc.comm_cost(g, op, D)
