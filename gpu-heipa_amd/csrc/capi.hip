@@ -139,6 +139,22 @@ int heipa_arena_release(heipa_ctx *ctx, uint64_t mark) {
 
 uint64_t heipa_arena_peak(const heipa_ctx *ctx) { return ctx ? ctx->c.arena.peak() : 0; }
 
+int heipa_download(heipa_ctx *ctx, void *host_dst, const void *dev_src, uint64_t bytes) {
+    return guarded(ctx, [&](Ctx &c) {
+        require(bytes == 0 || (host_dst && dev_src), "null argument");
+        copy_d2h(c, host_dst, dev_src, bytes);
+        c.sync();
+    });
+}
+
+int heipa_upload(heipa_ctx *ctx, void *dev_dst, const void *host_src, uint64_t bytes) {
+    return guarded(ctx, [&](Ctx &c) {
+        require(bytes == 0 || (dev_dst && host_src), "null argument");
+        copy_h2d(c, dev_dst, host_src, bytes);
+        c.sync();
+    });
+}
+
 int heipa_edge_cut(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, int64_t *cut) {
     return guarded(ctx, [&](Ctx &c) {
         check_graph(g);
@@ -342,6 +358,13 @@ int heipa_prof_enable(heipa_ctx *ctx, int on) {
     return guarded(ctx, [&](Ctx &c) {
         c.sync();
         c.prof_on = on != 0;
+    });
+}
+
+int heipa_prof_filter(heipa_ctx *ctx, const char *prefix) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.sync();
+        c.prof_filter = prefix ? prefix : "";
     });
 }
 

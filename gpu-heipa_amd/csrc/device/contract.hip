@@ -186,7 +186,9 @@ __global__ __launch_bounds__(CB) void row_block_kernel(DGraph g, const u32 *__re
     __shared__ u32 red[CB / WAVE];
     const u32 c = list[blockIdx.x];
     const u32 len = ub[c];
-    u32 P = 128;
+    // P is a power of two and a multiple of the block size, so every thread's
+    // slot in the dedupe loop below holds a sorted key or PAD
+    u32 P = CB;
     while (P < len) P <<= 1;
     // gather: walk the members' adjacency ranges in order
     {
@@ -307,14 +309,20 @@ __global__ void hub_offsets_kernel(const u32 *__restrict__ list, u32 nh, const u
 __global__ __launch_bounds__(CB) void copy_rows_kernel(u32 nc, const u32 *__restrict__ uboff,
                                                        const u32 *__restrict__ cxadj, const u32 *__restrict__ tkey,
                                                        const s32 *__restrict__ tw, u32 *__restrict__ cadj,
-                                                       s32 *__restrict__ cew) {
+                                                       s32 *__restrict__ cew, u32 *__restrict__ bad) {
     constexpr int W = 16;
     const u32 l = threadIdx.x % W;
     const u64 groups = (u64) gridDim.x * (CB / W);
     for (u64 c = (u64) blockIdx.x * (CB / W) + threadIdx.x / W; c < nc; c += groups) {
         const u32 src = uboff[c], dst = cxadj[c], len = cxadj[c + 1] - dst;
+        if (len > uboff[c + 1] - src) {
+            *bad = 1;
+            continue;
+        }
         for (u32 x = l; x < len; x += W) {
-            cadj[dst + x] = tkey[src + x];
+            const u32 key = tkey[src + x];
+            if (key >= nc || key == (u32) c) *bad = 1;
+            cadj[dst + x] = key;
             cew[dst + x] = tw[src + x];
         }
     }
@@ -428,13 +436,17 @@ CoarseGraph contract(Ctx &c, const DGraph &fine, const GraphMeta &meta, const u3
     c.sync();
     std::memcpy(hm, c.pinned, 8);
     const u32 mc = hm[0];
+    if (mc > fine.m) throw Error(HEIPA_ERR_DEVICE, "contraction produced more coarse edges than fine edges");
     if (hm[1]) throw Error(HEIPA_ERR_OVERFLOW, "coarse weight exceeds the s32 range of the graph format");
     u32 *cadj = (u32 *) c.arena.push_front(4ull * (mc ? mc : 1));
     s32 *cew = (s32 *) c.arena.push_front(4ull * (mc ? mc : 1));
     const double write_bytes = 4.0 * (nc + 1) + 8.0 * mc + 4.0 * nc;
     launch(c, "contract_copy", read_bytes + write_bytes, dim3(grid_for(c, nc, CB / 16, 8)), dim3(CB), 0,
            copy_rows_kernel, nc, (const u32 *) uboff, (const u32 *) cxadj, (const u32 *) tkey, (const s32 *) tw, cadj,
-           cew);
+           cew, scal + 9);
+    u32 bad;
+    readback(c, &bad, scal + 9, 1);
+    if (bad) throw Error(HEIPA_ERR_DEVICE, "contraction produced an inconsistent coarse graph");
     out.g.n = nc;
     out.g.m = mc;
     out.g.gw = fine.gw;

@@ -159,6 +159,7 @@ struct Ctx {
     std::string last_error;
     // profiling
     bool prof_on = false;
+    std::string prof_filter;  // record only kernels whose name starts with this (empty: all)
     std::vector<std::string> prof_names;
     std::map<std::string, size_t> prof_index;
     std::vector<KernelStat> prof_stats;
@@ -218,14 +219,15 @@ inline void launch(Ctx &c, const char *name, double bytes, dim3 grid, dim3 block
                    void (*kernel)(KArgs...), Args... args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
     hipEvent_t ea = nullptr, eb = nullptr;
-    if (c.prof_on) {
+    const bool timed = c.prof_on && (c.prof_filter.empty() || std::strncmp(name, c.prof_filter.c_str(), c.prof_filter.size()) == 0);
+    if (timed) {
         ea = c.get_event();
         eb = c.get_event();
         HIP_CHECK(hipEventRecord(ea, c.stream));
     }
     kernel<<<grid, block, shmem, c.stream>>>(static_cast<KArgs>(args)...);
     HIP_CHECK(hipGetLastError());
-    if (c.prof_on) {
+    if (timed) {
         HIP_CHECK(hipEventRecord(eb, c.stream));
         c.pending.push_back({c.prof_slot(name), ea, eb, bytes});
         if (c.pending.size() > 4096) c.sync();

@@ -173,8 +173,16 @@ __global__ __launch_bounds__(256) void block_weights_kernel(u32 n, const u32 *__
 __global__ void validate_vertices_kernel(DGraph g, u32 *flags) {
     for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < g.n; u += (u64) gridDim.x * blockDim.x) {
         const u32 b = g.xadj[u], e = g.xadj[u + 1];
-        if (b > e || e > g.m) atomicOr(flags, 1u);
+        if (b > e || e > g.m) {
+            atomicOr(flags, 1u);
+            continue;
+        }
         if (g.vw && g.vw[u] < 0) atomicOr(flags, 8u);
+        for (u32 j = b; j < e; ++j)
+            if (g.adj[j] == (u32) u) {
+                atomicOr(flags, 16u);
+                break;
+            }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (g.xadj[0] != 0 || g.xadj[g.n] != g.m)) atomicOr(flags, 1u);
 }
@@ -211,6 +219,7 @@ void validate_graph(Ctx &c, const DGraph &g) {
     if (f & 2u) throw Error(HEIPA_ERR_INVALID, "invalid graph: neighbour id >= n");
     if (f & 4u) throw Error(HEIPA_ERR_INVALID, "invalid graph: edge weights must be positive");
     if (f & 8u) throw Error(HEIPA_ERR_INVALID, "invalid graph: vertex weights must be non-negative");
+    if (f & 16u) throw Error(HEIPA_ERR_INVALID, "invalid graph: self loops are not allowed");
 }
 
 void validate_part(Ctx &c, const u32 *part, u32 n, u32 k) {

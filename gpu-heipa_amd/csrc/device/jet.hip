@@ -114,8 +114,10 @@ __device__ __forceinline__ void append(bool pred, u32 val, u32 *list, unsigned l
 }
 
 // ---------------------------------------------------------------- LDS block connectivity
-// per group: cnt[k] u64, touched[k] u32, tcount u32 (+pad)
-__host__ __device__ inline size_t conn_group_bytes(u32 k) { return 8ull * k + 4ull * k + 16; }
+// per group: cnt[k] u64, touched[k] u32, tcount u32; every group region is
+// padded to 16 bytes so the next group's u64 counters stay 8-byte aligned for
+// the 64-bit LDS atomics (12k + 16 is only a multiple of 8 for even k)
+__host__ __device__ inline size_t conn_group_bytes(u32 k) { return (12ull * k + 16 + 15) & ~(size_t) 15; }
 
 struct GroupConn {
     unsigned long long *cnt;

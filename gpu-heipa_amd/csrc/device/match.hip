@@ -506,7 +506,7 @@ u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping
         if (done) break;
         len_bound = len;
         batch = batch < 8 ? batch * 2 : 8;
-        require(round < 100000, "matching did not converge");
+        require(round < 20000, "matching did not converge (is the graph symmetric?)");
     }
     st.rounds = exact_rounds;
 

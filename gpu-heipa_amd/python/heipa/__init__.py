@@ -75,6 +75,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FileNotFoundError(f"{LIB_PATH} not built; run __graft_entry__.build() or make -C gpu-heipa_amd")
+        # torch bundles its own HIP runtime under an unversioned file name.
+        # Loading it first lets libheipa's libamdhip64.so.7 dependency bind to
+        # that same runtime; two HIP/HSA runtimes in one process cannot share
+        # the device.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         vp, c_int, c_u32, c_u64, c_dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double
         GP = ctypes.POINTER(Graph)
@@ -86,6 +91,8 @@ def lib():
             "heipa_arena_mark": (c_u64, [vp]),
             "heipa_arena_release": (c_int, [vp, c_u64]),
             "heipa_arena_peak": (c_u64, [vp]),
+            "heipa_download": (c_int, [vp, vp, vp, c_u64]),
+            "heipa_upload": (c_int, [vp, vp, vp, c_u64]),
             "heipa_edge_cut": (c_int, [vp, GP, vp, s64p]),
             "heipa_comm_cost": (c_int, [vp, GP, vp, vp, c_u32, s64p]),
             "heipa_distance_matrix": (c_int, [u32p, s32p, c_u32, s32p]),
@@ -102,6 +109,7 @@ def lib():
             "heipa_adaptive_imbalance": (c_dbl, [c_dbl, ctypes.c_int64, c_u32, ctypes.c_int64, c_u32, c_u64]),
             "heipa_prof_enable": (c_int, [vp, c_int]),
             "heipa_prof_reset": (c_int, [vp]),
+            "heipa_prof_filter": (c_int, [vp, ctypes.c_char_p]),
             "heipa_prof_count": (c_u32, [vp]),
             "heipa_prof_get": (c_int, [vp, c_u32, ctypes.c_char_p, c_u32, ctypes.POINTER(c_u64),
                                        ctypes.POINTER(c_dbl), ctypes.POINTER(c_dbl)]),
@@ -359,7 +367,7 @@ class Context:
         out = Graph()
         self._check("heipa_contract", self.L.heipa_contract(self.h, ctypes.byref(g.desc), mapping.data_ptr(), nc,
                                                             ctypes.byref(out)))
-        hg = _download_graph(out)
+        hg = self.download_graph(out)
         self._check("heipa_arena_release", self.L.heipa_arena_release(self.h, mark))
         return hg
 
@@ -409,9 +417,32 @@ class Context:
                                                                     int(ultra), part.data_ptr()))
         return part[:g.n]
 
+    def download(self, host_array: np.ndarray, dev_ptr):
+        self._check("heipa_download", self.L.heipa_download(self.h, host_array.ctypes.data, dev_ptr, host_array.nbytes))
+        return host_array
+
+    def download_graph(self, desc: Graph) -> HostGraph:
+        """Host copy of a device heipa_graph."""
+        n, m = desc.n, desc.m
+
+        def dl(ptr, count, dtype):
+            buf = np.zeros(count, dtype)
+            if count and ptr:
+                self.download(buf, ptr)
+            return buf
+
+        xadj = dl(desc.xadj, n + 1, np.uint32)
+        adjncy = dl(desc.adjncy, m, np.uint32)
+        adjwgt = dl(desc.adjwgt, m, np.int32) if desc.adjwgt else None
+        vwgt = dl(desc.vwgt, n, np.int32) if desc.vwgt else None
+        return HostGraph(xadj, adjncy, adjwgt, vwgt)
+
     # per-kernel timing
     def prof_enable(self, on=True):
         self._check("heipa_prof_enable", self.L.heipa_prof_enable(self.h, int(on)))
+
+    def prof_filter(self, prefix=""):
+        self._check("heipa_prof_filter", self.L.heipa_prof_filter(self.h, prefix.encode()))
 
     def prof_reset(self):
         self._check("heipa_prof_reset", self.L.heipa_prof_reset(self.h))
@@ -426,42 +457,6 @@ class Context:
                                                                 ctypes.byref(by)))
             out[buf.value.decode()] = {"launches": ln.value, "ms": ms.value, "bytes": by.value}
         return out
-
-
-def _download_graph(desc: Graph) -> HostGraph:
-    """Copies a device heipa_graph into a HostGraph via the HIP runtime of torch."""
-    torch = _torch()
-    n, m = desc.n, desc.m
-
-    def dl(ptr, count, dtype):
-        if count == 0 or not ptr:
-            return np.zeros(count, dtype)
-        buf = np.empty(count, dtype)
-        # hipMemcpy through torch: wrap the device pointer with a tensor view is not
-        # possible without ownership, so use the HIP runtime directly.
-        _hip_memcpy_d2h(buf.ctypes.data, ptr, buf.nbytes)
-        return buf
-
-    torch.cuda.synchronize()
-    xadj = dl(desc.xadj, n + 1, np.uint32)
-    adjncy = dl(desc.adjncy, m, np.uint32)
-    adjwgt = dl(desc.adjwgt, m, np.int32) if desc.adjwgt else None
-    vwgt = dl(desc.vwgt, n, np.int32) if desc.vwgt else None
-    return HostGraph(xadj, adjncy, adjwgt, vwgt)
-
-
-_hip = None
-
-
-def _hip_memcpy_d2h(dst, src, nbytes):
-    global _hip
-    if _hip is None:
-        _hip = ctypes.CDLL("libamdhip64.so")
-        _hip.hipMemcpy.restype = ctypes.c_int
-        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    rc = _hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, 2)  # hipMemcpyDeviceToHost
-    if rc != 0:
-        raise HeipaError("hipMemcpy", ERR_DEVICE)
 
 
 def to_numpy_u32(t):

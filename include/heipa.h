@@ -90,6 +90,10 @@ void *heipa_ctx_stream(heipa_ctx *ctx);          /* the hipStream_t all work run
 uint64_t heipa_arena_mark(const heipa_ctx *ctx); /* front-stack depth, for heipa_arena_release */
 int heipa_arena_release(heipa_ctx *ctx, uint64_t mark);
 uint64_t heipa_arena_peak(const heipa_ctx *ctx); /* peak bytes in use */
+/* copies between device and host memory on the context stream (synchronous);
+ * lets callers without their own HIP runtime handle read results */
+int heipa_download(heipa_ctx *ctx, void *host_dst, const void *dev_src, uint64_t bytes);
+int heipa_upload(heipa_ctx *ctx, void *dev_dst, const void *host_src, uint64_t bytes);
 
 /* ---- evaluators */
 /* edge_cut<uew>(const Graph&, const Partition&, exec) — src/utility/edge_cut.h:71-90 (64-bit sum, halved) */
@@ -145,6 +149,8 @@ double heipa_adaptive_imbalance(double global_imbalance, int64_t global_g_weight
 
 /* ---- per-kernel timing (HIP events on the context stream) */
 int heipa_prof_enable(heipa_ctx *ctx, int on);
+/* time only kernels whose name starts with `prefix` (NULL or "" = all) */
+int heipa_prof_filter(heipa_ctx *ctx, const char *prefix);
 int heipa_prof_reset(heipa_ctx *ctx);
 uint32_t heipa_prof_count(heipa_ctx *ctx);
 /* algorithmic bytes = the compulsory HBM bytes of each launch, summed */

@@ -84,11 +84,11 @@ def _trace_tuple(t):
     return (t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
 
 
-@pytest.mark.parametrize("ultra,level,skew", [(False, 0, 0.0), (False, 1, 0.0), (True, 0, 0.0), (False, 0, 0.6),
-                                              (True, 2, 0.9)])
-def test_jet(ctx, graph, ultra, level, skew):
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.0, 8), (False, 1, 0.0, 8), (True, 0, 0.0, 8),
+                                                (False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 0, 0.3, 3),
+                                                (True, 1, 0.0, 5), (False, 1, 0.7, 7)])
+def test_jet(ctx, graph, ultra, level, skew, k):
     name, g = graph
-    k = 8
     dg = _dev(g)
     # start from a coarse-ish partition: blocks of consecutive ids, perturbed
     part0 = (np.arange(g.n, dtype=np.int64) * k // max(g.n, 1)).astype(np.uint32)
@@ -111,7 +111,7 @@ def test_jet(ctx, graph, ultra, level, skew):
     assert orc.edge_cut(g, gp) == gcut
 
 
-@pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False)])
+@pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False), (3, False), (7, True)])
 def test_partition(ctx, graph, k, ultra):
     name, g = graph
     dg = _dev(g)
