@@ -340,10 +340,10 @@ void launch_row_small(Ctx &c, const DGraph &g, const u32 *map, const u32 *moff, 
     if (!len) return;
     const dim3 grid(grid_for(c, len, CB / W, 8)), block(CB);
     if (g.ew)
-        launch(c, "contract_rows_small", 0.0, grid, block, 0, row_small_kernel<W, false>, g, map, moff, mem, uboff,
+        launch(c, "contract_rows", 0.0, grid, block, 0, row_small_kernel<W, false>, g, map, moff, mem, uboff,
                list, len, tkey, tw, cnt, overflow);
     else
-        launch(c, "contract_rows_small", 0.0, grid, block, 0, row_small_kernel<W, true>, g, map, moff, mem, uboff,
+        launch(c, "contract_rows", 0.0, grid, block, 0, row_small_kernel<W, true>, g, map, moff, mem, uboff,
                list, len, tkey, tw, cnt, overflow);
 }
 
@@ -399,11 +399,11 @@ CoarseGraph contract(Ctx &c, const DGraph &fine, const GraphMeta &meta, const u3
     launch_row_small<64>(c, fine, map, moff, mem, uboff, tier_lists + 3ull * stride, tc[3], tkey, tw, cnt, scal + 8);
     if (tc[4]) {
         if (fine.ew)
-            launch(c, "contract_rows_block", 0.0, dim3(tc[4]), dim3(CB), 0, row_block_kernel<false>, fine, map,
+            launch(c, "contract_rows", 0.0, dim3(tc[4]), dim3(CB), 0, row_block_kernel<false>, fine, map,
                    (const u32 *) moff, (const u32 *) mem, (const u32 *) uboff, (const u32 *) ub,
                    (const u32 *) (tier_lists + 4ull * stride), tkey, tw, cnt, scal + 8);
         else
-            launch(c, "contract_rows_block", 0.0, dim3(tc[4]), dim3(CB), 0, row_block_kernel<true>, fine, map,
+            launch(c, "contract_rows", 0.0, dim3(tc[4]), dim3(CB), 0, row_block_kernel<true>, fine, map,
                    (const u32 *) moff, (const u32 *) mem, (const u32 *) uboff, (const u32 *) ub,
                    (const u32 *) (tier_lists + 4ull * stride), tkey, tw, cnt, scal + 8);
     }
@@ -437,11 +437,16 @@ CoarseGraph contract(Ctx &c, const DGraph &fine, const GraphMeta &meta, const u3
     std::memcpy(hm, c.pinned, 8);
     const u32 mc = hm[0];
     if (mc > fine.m) throw Error(HEIPA_ERR_DEVICE, "contraction produced more coarse edges than fine edges");
+    // row building: member adjacencies 4m(1+w), coarse ids of the neighbours
+    // (distinct fine vertices, at most 4n), member offsets and xadj 12n,
+    // per-row metadata 16nc, unique entries written 8mc
+    prof_bytes(c, "contract_rows", 4.0 * fine.m * (fine.ew ? 2 : 1) + std::min(4.0 * fine.m, 4.0 * n) + 12.0 * n +
+                                       16.0 * nc + 8.0 * mc);
     if (hm[1]) throw Error(HEIPA_ERR_OVERFLOW, "coarse weight exceeds the s32 range of the graph format");
     u32 *cadj = (u32 *) c.arena.push_front(4ull * (mc ? mc : 1));
     s32 *cew = (s32 *) c.arena.push_front(4ull * (mc ? mc : 1));
-    const double write_bytes = 4.0 * (nc + 1) + 8.0 * mc + 4.0 * nc;
-    launch(c, "contract_copy", read_bytes + write_bytes, dim3(grid_for(c, nc, CB / 16, 8)), dim3(CB), 0,
+    (void) read_bytes;
+    launch(c, "contract_copy", 16.0 * mc + 8.0 * (nc + 1), dim3(grid_for(c, nc, CB / 16, 8)), dim3(CB), 0,
            copy_rows_kernel, nc, (const u32 *) uboff, (const u32 *) cxadj, (const u32 *) tkey, (const s32 *) tw, cadj,
            cew, scal + 9);
     u32 bad;

@@ -212,6 +212,14 @@ struct Ctx {
     T *pinned_as() { return (T *) pinned; }
 };
 
+// Adds data-dependent algorithmic bytes to a timed kernel's total (for kernels
+// whose compulsory traffic is only known after they ran).
+inline void prof_bytes(Ctx &c, const char *name, double bytes) {
+    if (!c.prof_on) return;
+    if (!c.prof_filter.empty() && std::strncmp(name, c.prof_filter.c_str(), c.prof_filter.size()) != 0) return;
+    c.prof_stats[c.prof_slot(name)].bytes += bytes;
+}
+
 // Kernel launch on the context stream with optional event timing.
 // `bytes` is the launch's algorithmic (compulsory) HBM traffic.
 template<typename... KArgs, typename... Args>

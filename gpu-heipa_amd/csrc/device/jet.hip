@@ -81,6 +81,9 @@ enum Stat : u32 {
     ST_MAXW,
     ST_MHASH,
     ST_GHASH,
+    ST_MISS_DEG,  // sum of degrees of the rebuilt (missed) vertices
+    ST_CAND_DEG,  // sum of degrees of the afterburner candidates
+    ST_MOVE_DEG,  // sum of degrees of the moved vertices
     ST_COUNT
 };
 
@@ -278,10 +281,12 @@ __global__ void lp_miss_kernel(JetDev d, double ratio) {
     if (lane == 0) *gc.tcount = 0;
     __syncthreads();
     const u32 len = (u32) d.st[ST_MISS];
+    u64 degs = 0;
     for (u64 i0 = (u64) blockIdx.x * groups; i0 < len; i0 += (u64) gridDim.x * groups) {
         const u64 i = i0 + gi;
         const bool active = i < len;
         const u32 u = active ? d.miss[i] : 0;
+        if (active && lane == 0) degs += d.xadj[u + 1] - d.xadj[u];
         if (active) conn_accumulate(d, gc, u, lane, G);
         __syncthreads();
         bool cand;
@@ -292,6 +297,8 @@ __global__ void lp_miss_kernel(JetDev d, double ratio) {
         if (active && lane == 0) *gc.tcount = 0;
         append(cand, u, d.cand, &d.st[ST_CAND]);
     }
+    degs = wave_sum(degs);
+    if (lane_id() == 0 && degs) atomicAdd(&d.st[ST_MISS_DEG], (unsigned long long) degs);
 }
 
 // K1h: one workgroup per hub (hubs never enter the miss list)
@@ -309,6 +316,7 @@ __global__ void lp_hub_kernel(JetDev d, const u32 *hubs, double ratio) {
     bool cand;
     lp_eval<JB>(d, gc, u, threadIdx.x, ratio, red, true, cand);
     append(cand, u, d.cand, &d.st[ST_CAND]);
+    if (threadIdx.x == 0) atomicAdd(&d.st[ST_MISS_DEG], (unsigned long long) (d.xadj[u + 1] - d.xadj[u]));
 }
 
 // K2: afterburner (jet:269-328) over candidates, G lanes each; selected
@@ -335,12 +343,13 @@ __global__ void afterburner_kernel(JetDev d, int do_hash) {
     const u32 groups = blockDim.x / G;
     const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
     const u32 len = (u32) d.st[ST_CAND];
-    u64 gh = 0;
+    u64 gh = 0, degs = 0;
     for (u64 i0 = (u64) blockIdx.x * groups; i0 < len; i0 += (u64) gridDim.x * groups) {
         const u64 i = i0 + gi;
         const bool active = i < len;
         const u32 u = active ? d.cand[i] : 0;
         const bool hub = active && (d.xadj[u + 1] - d.xadj[u]) > d.hub_deg;
+        if (active && !hub && lane == 0) degs += d.xadj[u + 1] - d.xadj[u];
         s64 ch = (active && !hub) ? afterburner_change<G>(d, u, lane, G) : 0;
         ch = group_sum<G>(ch, red);
         bool sel = false;
@@ -356,6 +365,8 @@ __global__ void afterburner_kernel(JetDev d, int do_hash) {
         }
         append(sel, u, d.moves, &d.st[ST_MOVES]);
     }
+    degs = wave_sum(degs);
+    if (lane_id() == 0 && degs) atomicAdd(&d.st[ST_CAND_DEG], (unsigned long long) degs);
     if (do_hash) {
         gh = wave_sum(gh);
         if (lane_id() == 0 && gh) atomicAdd(&d.st[ST_GHASH], (unsigned long long) gh);
@@ -371,6 +382,7 @@ __global__ void afterburner_hub_kernel(JetDev d, const u32 *hubs, int do_hash) {
     ch = group_sum<JB>(ch, red);
     bool sel = false;
     if (threadIdx.x == 0) {
+        atomicAdd(&d.st[ST_CAND_DEG], (unsigned long long) (d.xadj[u + 1] - d.xadj[u]));
         if (do_hash) atomicAdd(&d.st[ST_GHASH], (unsigned long long) mix64d(mix64d(u) + (u64) ug));
         if (ug + ch >= 0) {
             d.lock[u] = 1;
@@ -409,7 +421,7 @@ __global__ void apply_kernel(JetDev d, int do_hash) {
     const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
     const u32 len = (u32) d.st[ST_MOVES];
     s64 acc = 0;
-    u64 mh = 0;
+    u64 mh = 0, degs = 0;
     for (u64 i0 = (u64) blockIdx.x * groups; i0 < len; i0 += (u64) gridDim.x * groups) {
         const u64 i = i0 + gi;
         const bool active = i < len;
@@ -418,6 +430,7 @@ __global__ void apply_kernel(JetDev d, int do_hash) {
         if (active && !hub) {
             acc += move_delta<G>(d, u, lane, G);
             if (lane == 0) {
+                degs += d.xadj[u + 1] - d.xadj[u];
                 const u32 ou = d.oldp[u], nu = d.dest[u];
                 const s64 w = vwt(d, u);
                 d.dcache[u] = NULL_PART;
@@ -430,6 +443,8 @@ __global__ void apply_kernel(JetDev d, int do_hash) {
     }
     acc = block_sum<JB>(acc, red);
     if (threadIdx.x == 0 && acc) atomicAdd(&d.st[ST_DELTA], (unsigned long long) acc);
+    degs = wave_sum(degs);
+    if (lane_id() == 0 && degs) atomicAdd(&d.st[ST_MOVE_DEG], (unsigned long long) degs);
     if (do_hash) {
         mh = wave_sum(mh);
         if (lane_id() == 0 && mh) atomicAdd(&d.st[ST_MHASH], (unsigned long long) mh);
@@ -444,6 +459,7 @@ __global__ void apply_hub_kernel(JetDev d, const u32 *hubs, int do_hash) {
     acc = block_sum<JB>(acc, red);
     if (threadIdx.x == 0) {
         if (acc) atomicAdd(&d.st[ST_DELTA], (unsigned long long) acc);
+        atomicAdd(&d.st[ST_MOVE_DEG], (unsigned long long) (d.xadj[u + 1] - d.xadj[u]));
         const u32 ou = d.oldp[u], nu = d.dest[u];
         const s64 w = vwt(d, u);
         d.dcache[u] = NULL_PART;
@@ -827,8 +843,9 @@ struct JetHost {
 
     void lp(double ratio) {
         begin_iteration();
-        const double lp_bytes = 4.0 * (g.n + 1) + 4.0 * g.m * (g.ew ? 2 : 1) + 4.0 * g.n + 8.0 * g.n;
-        launch(c, "jet_lp_cached", lp_bytes, dim3(grid_for(c, g.n, 256, 8)), dim3(256), 0, lp_cached_kernel, d);
+        // reads dest_cache, lock, gain_cache; writes dest, gain1 (all n-sized)
+        const double cached_bytes = 25.0 * g.n;
+        launch(c, "jet_lp_cached", cached_bytes, dim3(grid_for(c, g.n, 256, 8)), dim3(256), 0, lp_cached_kernel, d);
         HEIPA_DISPATCH_G(G, launch_lp_miss, ratio);
         if (meta.n_hubs)
             launch(c, "jet_lp_best_block", 0.0, dim3(meta.n_hubs), dim3(JB), conn_group_bytes(g_k()), lp_hub_kernel,
@@ -924,6 +941,27 @@ struct JetHost {
     }
 };
 
+// Algorithmic bytes of the data-dependent Jet kernels, from the counters of
+// the iteration (w = 1 when edge weights are stored):
+//   best block over misses M: 4(|M|+1)... per miss 8 (xadj) + 4 (own block)
+//     + 25 (lock, gain/dest caches, dest, gain1), per edge 4(1+w) + 4 (block of
+//     the neighbour, counted at most once per vertex: capped at 4n);
+//   afterburner over candidates C: per candidate 8 + 8 (own gain, blocks),
+//     per edge 4(1+w), neighbour gain/dest/block 16 per distinct vertex (cap 16n);
+//   apply over moves M: per move 16, per edge 4(1+w), neighbour mark/block and
+//     cache invalidation 16 per distinct vertex (cap 16n).
+void account_bytes(Ctx &c, const DGraph &g, u32 mode, const unsigned long long *rep) {
+    const double w = g.ew ? 2.0 : 1.0, n = g.n;
+    if (mode == 0) {
+        const double nm = (double) rep[ST_MISS], em = (double) rep[ST_MISS_DEG];
+        prof_bytes(c, "jet_lp_best_block", 37.0 * nm + 4.0 * w * em + std::min(4.0 * em, 4.0 * n));
+        const double nc = (double) rep[ST_CAND], ec = (double) rep[ST_CAND_DEG];
+        prof_bytes(c, "jet_afterburner", 16.0 * nc + 4.0 * w * ec + std::min(16.0 * ec, 16.0 * n));
+    }
+    const double nmv = (double) rep[ST_MOVES], emv = (double) rep[ST_MOVE_DEG];
+    prof_bytes(c, "jet_apply", 16.0 * nmv + 4.0 * w * emv + std::min(16.0 * emv, 16.0 * n));
+}
+
 } // namespace
 
 JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, u32 k, s64 lmax, bool ultra,
@@ -1017,6 +1055,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
             }
             readback(c, rep, h.report, ST_COUNT);
             const u32 n_moves = (u32) rep[ST_MOVES];
+            if (c.prof_on) account_bytes(c, g, mode, rep);
             if (mode == 0) {
                 n_cand = (u32) rep[ST_CAND];
                 bal = 0;
