@@ -84,6 +84,11 @@ int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out) {
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         c.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        {
+            int lds = 0;
+            if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+                c.max_lds = (size_t) lds;
+        }
         HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         if (arena_bytes == 0) {
             size_t free_b = 0, total_b = 0;
@@ -93,7 +98,7 @@ int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out) {
         arena_bytes = (arena_bytes + 255) & ~(uint64_t) 255;
         HIP_CHECK(hipMalloc(&c.arena_mem, arena_bytes));
         c.arena.init(c.arena_mem, arena_bytes);
-        c.pinned_bytes = 1 << 16;
+        c.pinned_bytes = 1 << 18;
         HIP_CHECK(hipHostMalloc(&c.pinned, c.pinned_bytes, hipHostMallocDefault));
     } catch (const Error &e) {
         std::fprintf(stderr, "heipa_ctx_create: %s\n", e.what());
@@ -358,6 +363,14 @@ int heipa_prof_enable(heipa_ctx *ctx, int on) {
     return guarded(ctx, [&](Ctx &c) {
         c.sync();
         c.prof_on = on != 0;
+    });
+}
+
+int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.sync();
+        if (option == HEIPA_OPT_RESOLVE_CAP) c.resolve_cap = value > 0xFFFFFFFFull ? ~0u : (u32) value;
+        else require(false, "unknown option");
     });
 }
 

@@ -150,6 +150,8 @@ struct KernelStat {
 struct Ctx {
     int device = 0;
     int num_cus = 256;
+    size_t max_lds = 65536;  // LDS bytes one workgroup may allocate
+    u32 resolve_cap = ~0u;   // HEIPA_OPT_RESOLVE_CAP
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;

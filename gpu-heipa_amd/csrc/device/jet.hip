@@ -1,4 +1,5 @@
-// Jet refinement on the GPU, deterministic.
+// Jet refinement on the GPU, deterministic, with the refinement loop's control
+// flow kept on the device.
 //
 // Follows jet_refine (src/refinement/jet_label_propagation.h:926-1033) with
 // jet_lp (:182-354), rebalance_weak (:592-751), rebalance_strong (:356-590) and
@@ -12,17 +13,29 @@
 //   * the cut change of a move set is computed exactly in 64 bits from the old
 //     and new blocks of each moved vertex and its neighbours; it equals the
 //     reference's (c1 + c2) / 2 (jet:763-923).
-//   * minibucket offsets: candidates are sorted by minibucket with a stable
-//     radix sort over an id-ordered list, so a vertex's offset is the weight of
-//     smaller ids in its minibucket; strong-rebalance overflow (jet:576-583) is
-//     resolved serially in id order.
+//   * minibucket offsets: a vertex's offset inside its minibucket is the weight
+//     of smaller ids in it; strong-rebalance overflow (jet:576-583) is resolved
+//     serially in id order.  Only candidates whose offset can decide their
+//     selection are sorted: one workgroup sorts them by (minibucket, id) in
+//     LDS; longer lists go through a device-wide radix sort.
 //   * gain_bucket (jet:147-180): float quotient as in the reference, then an
 //     exact comparison against float thresholds of powers of 1.5.
-// One host readback per iteration drives the reference's control flow.
+//   * control: the loop of jet_refine (ratio index, iteration counters,
+//     weak/strong switching, best-partition tracking) runs in the last
+//     workgroup of each iteration's apply launch.  Every kernel reads the
+//     control block first and returns at once when the loop has ended, so the
+//     host enqueues iterations in batches and reads the control block once per
+//     batch instead of once per iteration.
+// Launches per iteration: sweep (LP best block | rebalance score), buckets
+// (rebalance only: per-block prefix of the minibucket totals), post
+// (afterburner | rebalance classify), resolve (rebalance only), apply, and a
+// one-workgroup control launch (block weights, counters, loop state).  No
+// kernel synchronises across workgroups: every hand-off is a launch boundary.
 #include "eval.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace heipa {
 namespace {
@@ -36,7 +49,16 @@ constexpr double PHI = 0.999;
 constexpr u32 MAX_SECTIONS = 128;
 constexpr u32 MAX_BUCKETS = 50;
 constexpr int MID_BUCKETS = 25;
-constexpr int JB = 256;
+constexpr int JB = 256;                  // threads of the group-per-vertex kernels
+constexpr int RB = 1024;                 // threads of the resolve workgroup
+constexpr u32 RESOLVE_CAP = 8192;        // resolve entries sorted in LDS (power of two) ...
+constexpr u32 RESOLVE_CAP_SMALL = 2048;  // ... or this many where a workgroup gets at most 64 KiB of LDS
+constexpr u32 OVF_CAP = 512;             // strong-rebalance overflow entries kept in LDS
+constexpr u32 ES_CAP = 1025;             // eviction chunk bounds kept in LDS (k + 1)
+constexpr u32 TRACE_RING = 256;          // trace records kept on the device between host reads
+constexpr u32 MAX_RATIOS = 17;
+constexpr size_t SWEEP_LDS = 48 * 1024;  // block-connectivity counters per sweep workgroup
+constexpr u32 BATCH_MAX = 8;             // iterations enqueued per host read of the control block
 
 // float thresholds: LO[k] = least float >= 1.5^-k, HI[k] = greatest float <= 1.5^k
 __constant__ float kLo[24] = {
@@ -71,41 +93,93 @@ __device__ __forceinline__ u64 mix64d(u64 z) {
     return z ^ (z >> 31);
 }
 
-// live counters / report (device)
+// Counters of the iteration in flight.  Lists have one region per sweep
+// workgroup, filled without global atomics (an LDS counter per workgroup),
+// plus an extra list for the few producers outside the sweep grid (hub
+// workgroups, resolve, global path), appended with global atomics; st[]
+// holds the extra lists' lengths and the hub workgroups' statistics, and the
+// per-workgroup statistics go to a partial array.  The control launch sums
+// both into the iteration's totals and clears st[].
+enum Part : u32 { PT_MISS = 0, PT_MISS_DEG, PT_CAND_DEG, PT_GHASH, PT_DELTA, PT_MOVE_DEG, PT_MHASH, PT_COUNT = 8 };
 enum Stat : u32 {
-    ST_MISS = 0,
-    ST_CAND,
-    ST_MOVES,
-    ST_OVER,
-    ST_DELTA,  // s64 raw delta (twice the cut change)
-    ST_MAXW,
+    ST_MISS = 0,  // vertices rebuilt (LP) / scored (rebalance)
+    ST_CAND,      // LP candidates / rebalance candidates (extra list: length)
+    ST_MOVES,     // moves (extra list: length)
+    ST_RLIST,     // rebalance resolve list length
+    ST_DELTA,     // s64 raw cut delta (twice the cut change)
     ST_MHASH,
     ST_GHASH,
-    ST_MISS_DEG,  // sum of degrees of the rebuilt (missed) vertices
+    ST_MISS_DEG,  // sum of degrees of the rebuilt / scored vertices
     ST_CAND_DEG,  // sum of degrees of the afterburner candidates
     ST_MOVE_DEG,  // sum of degrees of the moved vertices
     ST_COUNT
 };
 
+// The state of jet_refine's loop (jet:960-1030), in device memory.
+struct JetCtl {
+    unsigned long long st[ST_COUNT];
+    s64 cur_cut, cur_maxw, best_cut, best_maxw;
+    double ratio;     // ratio of the current ratio index
+    double bytes[5];  // algorithmic bytes of the sweep, buckets, post, resolve and apply launches
+    u32 ri, it, bal, mode;
+    u32 active;       // 0 once the loop has ended
+    u32 stalled;      // the resolve list did not fit: the host runs the global path
+    u32 save;         // the partition after the last iteration is the best so far
+    u32 iters;        // iterations that changed the partition
+    u32 n_trace, stamp;
+};
+
 struct JetDev {
     u32 n, k;
-    s64 lmax;
+    s64 lmax, opt;  // opt = ceil(W / k)
     const u32 *xadj, *adj;
     const s32 *ew, *vw;
-    u32 *work, *dest, *dcache, *mark, *oldp;
+    u32 *work, *dest, *dcache, *mark, *oldp, *part;
     s64 *gcache, *gain1;
     u8 *lock;
-    u32 *miss, *cand, *moves;
+    u32 *cand, *moves;      // regions of cap / cap_mv entries, one per sweep workgroup
+    u32 *cand_n, *moves_n;  // region lengths
+    u32 *cand_x, *moves_x;  // extra lists (lengths in st[ST_CAND], st[ST_MOVES])
+    unsigned long long *pstat;  // PT_COUNT partial statistics per sweep workgroup
+    u32 nreg, cap, cap_mv;
     s64 *bw;
-    unsigned long long *st;  // Stat slots
-    u32 stamp;
-    u32 hub_deg;
+    JetCtl *ctl;
+    heipa_jet_trace *trace;  // ring of TRACE_RING records; null: no trace, no hashes
+    const double *ratios;
+    u32 n_ratios, hub_deg;
+    u32 rcap;  // resolve entries sorted in LDS (<= the kernel's capacity)
+    // rebalancing
+    u32 sections, width, t;
+    u32 *gid;           // minibucket per vertex
+    s64 *btot;          // t + 1 minibucket totals, zero between iterations
+    s64 *Q, *R;         // t: exclusive / inclusive prefix of btot inside each block's range
+    s64 *Tseg;          // k: total of each block's range
+    u64 *rlist;         // resolve list: minibucket << 32 | vertex
+    s64 *evict_adjust;  // k
+    s64 *evict_start;   // k + 1
+    s64 *counter;       // strong-rebalance overflow position
 };
 
 __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64) d.vw[u] : 1; }
 __device__ __forceinline__ s64 ewt(const JetDev &d, u32 j) { return d.ew ? (s64) d.ew[j] : 1; }
+__device__ __forceinline__ u32 degree(const JetDev &d, u32 u) { return d.xadj[u + 1] - d.xadj[u]; }
+__device__ __forceinline__ bool live(const JetCtl *ctl) { return ctl->active && !ctl->stalled; }
 
-// wave-aggregated append to a device list
+// eviction threshold of rebalance_strong (jet:363) / rebalance_weak (jet:600-604)
+__device__ __forceinline__ s64 max_block_weight(const JetDev &d, bool strong) {
+    const s64 a = (s64) ((double) d.lmax * 0.99);
+    if (strong) return d.opt + 1 > a ? d.opt + 1 : a;
+    return a < d.lmax - 100 ? d.lmax - 100 : a;
+}
+
+// orders the LDS traffic of one wave (lane groups never span waves)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// wave-aggregated appends to device lists
 __device__ __forceinline__ void append(bool pred, u32 val, u32 *list, unsigned long long *counter) {
     const u64 bal = __ballot(pred);
     if (!bal) return;
@@ -114,6 +188,44 @@ __device__ __forceinline__ void append(bool pred, u32 val, u32 *list, unsigned l
     if (lane_id() == first) base = (u32) atomicAdd(counter, (unsigned long long) __popcll(bal));
     base = __shfl(base, (int) first, WAVE);
     if (pred) list[base + (u32) __popcll(bal & lanemask_lt())] = val;
+}
+__device__ __forceinline__ void append64(bool pred, u64 val, u64 *list, unsigned long long *counter) {
+    const u64 bal = __ballot(pred);
+    if (!bal) return;
+    const u32 first = (u32) __builtin_ctzll(bal);
+    u32 base = 0;
+    if (lane_id() == first) base = (u32) atomicAdd(counter, (unsigned long long) __popcll(bal));
+    base = __shfl(base, (int) first, WAVE);
+    if (pred) list[base + (u32) __popcll(bal & lanemask_lt())] = val;
+}
+
+// wave-aggregated append to this workgroup's region (LDS counter)
+__device__ __forceinline__ void append_local(bool pred, u32 val, u32 *region, u32 *s_cnt) {
+    const u64 bal = __ballot(pred);
+    if (!bal) return;
+    const u32 first = (u32) __builtin_ctzll(bal);
+    u32 base = 0;
+    if (lane_id() == first) base = atomicAdd(s_cnt, (u32) __popcll(bal));
+    base = __shfl(base, (int) first, WAVE);
+    if (pred) region[base + (u32) __popcll(bal & lanemask_lt())] = val;
+}
+
+// sum over a workgroup of any multiple-of-64 size; the result is valid in
+// thread 0; `lds` holds blockDim.x / 64 entries
+__device__ __forceinline__ u64 block_total(u64 v, u64 *lds) {
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane_id() == 0) lds[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    u64 t = 0;
+    if (threadIdx.x == 0)
+        for (u32 w = 0; w < blockDim.x / WAVE; ++w) t += lds[w];
+    return t;
+}
+
+__device__ __forceinline__ void wave_add(unsigned long long *ctr, u64 v) {
+    v = wave_sum(v);
+    if (lane_id() == 0 && v) atomicAdd(ctr, (unsigned long long) v);
 }
 
 // ---------------------------------------------------------------- LDS block connectivity
@@ -150,7 +262,7 @@ __device__ __forceinline__ void conn_reset(const GroupConn &gc, u32 T, u32 lane,
     for (u32 i = lane; i < T; i += G) gc.cnt[gc.touched[i]] = 0;
 }
 
-// lexicographic best (conn desc, block asc) with a validity flag
+// lexicographic best (conn desc, block asc)
 struct Best {
     s64 c;
     u32 b;
@@ -205,37 +317,12 @@ __device__ __forceinline__ T group_sum(T x, T *red) {
     }
 }
 
-// ---------------------------------------------------------------- LP kernels
-// K1a: cached vertices take their cached decision and are filtered; misses are listed
-__global__ void lp_cached_kernel(JetDev d) {
-    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < d.n; i0 += (u64) gridDim.x * blockDim.x) {
-        const u64 u = i0 + threadIdx.x;
-        bool is_miss = false, is_cand = false;
-        if (u < d.n) {
-            const u32 dc = d.dcache[u];
-            if (dc == NULL_PART) {
-                // hub misses are handled by lp_hub_kernel
-                is_miss = (d.xadj[u + 1] - d.xadj[u]) <= d.hub_deg;
-            } else {
-                d.dest[u] = dc;
-                if (dc != NO_MOVE && d.lock[u] == 0) {
-                    is_cand = true;
-                    d.gain1[u] = d.gcache[u];
-                } else {
-                    d.gain1[u] = GAIN_MIN;
-                    d.lock[u] = 0;
-                }
-            }
-        }
-        append(is_miss, (u32) u, d.miss, &d.st[ST_MISS]);
-        append(is_cand, (u32) u, d.cand, &d.st[ST_CAND]);
-    }
-}
-
-// shared evaluation of an LP miss (best_block :200-236 + filter :247-258)
+// ---------------------------------------------------------------- LP evaluation
+// best_block (:200-236) + filter (:247-258) of one rebuilt vertex; lane 0 of
+// the group writes the results; returns whether u is a candidate
 template<int G>
-__device__ __forceinline__ void lp_eval(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, double ratio,
-                                        Best *red, bool active, bool &cand_out) {
+__device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, double ratio,
+                                        Best *red, bool active) {
     const u32 T = active ? *gc.tcount : 0;
     const u32 own_b = active ? d.work[u] : 0;
     Best best{0, NO_MOVE};
@@ -246,7 +333,7 @@ __device__ __forceinline__ void lp_eval(const JetDev &d, const GroupConn &gc, u3
         if (best_better(x, best)) best = x;
     }
     best = best_reduce<G>(best, red);
-    cand_out = false;
+    bool cand = false;
     if (active && lane == 0) {
         const s64 own = (s64) gc.cnt[own_b];
         u32 best_id = best.b;
@@ -261,67 +348,311 @@ __device__ __forceinline__ void lp_eval(const JetDev &d, const GroupConn &gc, u3
         d.dest[u] = best_id;
         if (best_id != NO_MOVE && d.lock[u] == 0) {
             d.gain1[u] = gain;
-            cand_out = true;
+            cand = true;
         } else {
             d.gain1[u] = GAIN_MIN;
             d.lock[u] = 0;
         }
     }
+    return cand;
 }
 
-// K1b: G lanes per listed miss
+// ---------------------------------------------------------------- rebalance evaluation
+// per-workgroup view of the block weights: the underloaded blocks in ascending
+// order (weak, jet:614-632) or the largest remaining capacity below the
+// threshold (strong, jet:374-388; -1 if none)
+struct RebShared {
+    s64 max_vwgt;
+    u32 n_under, pad;
+    u32 under[1];  // k entries
+};
+
+// computed by the first wave, 64 blocks per step
+__device__ __forceinline__ void reb_prologue(const JetDev &d, bool strong, s64 mbw, RebShared *rs) {
+    if (threadIdx.x < WAVE) {
+        const u32 lane = threadIdx.x;
+        if (!strong) {
+            u32 c = 0;
+            for (u32 b0 = 0; b0 < d.k; b0 += WAVE) {
+                const u32 b = b0 + lane;
+                const bool f = b < d.k && d.bw[b] < mbw;
+                const u64 bal = __ballot(f);
+                if (f) rs->under[c + (u32) __popcll(bal & lanemask_lt())] = b;
+                c += (u32) __popcll(bal);
+            }
+            if (lane == 0) rs->n_under = c;
+        } else {
+            s64 mx = -1;
+            for (u32 b = lane; b < d.k; b += WAVE)
+                if (d.bw[b] < mbw && mbw - d.bw[b] > mx) mx = mbw - d.bw[b];
+            mx = wave_max(mx);
+            if (lane == 0) rs->max_vwgt = mx;
+        }
+    }
+    __syncthreads();
+}
+
+// vertices considered for eviction (weak jet:641-646, strong jet:394-398)
+__device__ __forceinline__ bool reb_eligible(const JetDev &d, bool strong, const RebShared *rs, u32 u) {
+    const u32 ub = d.work[u];
+    const s64 bwu = d.bw[ub];
+    const s64 wu = vwt(d, u);
+    if (!strong) return bwu > d.lmax && (double) wu < 1.5 * (double) (bwu - d.opt);
+    const s64 mv = rs->max_vwgt;
+    return mv >= 0 && bwu > d.lmax && wu <= 2 * mv && wu < 2 * (bwu - d.opt);
+}
+
+// weak best block (jet:658-694) / strong score (jet:399-435) of one vertex;
+// lane 0 records dest (weak), the minibucket and its weight; returns whether
+// u is a candidate
 template<int G>
-__global__ void lp_miss_kernel(JetDev d, double ratio) {
+__device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, const RebShared *rs,
+                                         const GroupConn &gc, u32 u, u32 lane, Best *red, s64 *red2, bool active) {
+    const u32 T = active ? *gc.tcount : 0;
+    const u32 own_b = active ? d.work[u] : 0;
+    bool listed = false;
+    if (!strong) {
+        Best best{0, NO_MOVE};
+        for (u32 i = lane; i < T; i += G) {
+            const u32 b = gc.touched[i];
+            if (d.bw[b] >= mbw) continue;
+            Best x{(s64) gc.cnt[b], b};
+            if (best_better(x, best)) best = x;
+        }
+        best = best_reduce<G>(best, red);
+        if (active && lane == 0) {
+            const s64 own = (s64) gc.cnt[own_b];
+            u32 dst;
+            s64 gain;
+            if (best.b != NO_MOVE && best.c > 0) {
+                dst = best.b;
+                gain = best.c - own;
+            } else {
+                const u32 nu = rs->n_under;
+                dst = nu ? rs->under[u % nu] : own_b;
+                gain = -own;
+            }
+            d.dest[u] = dst;
+            if (dst != own_b) {
+                const u32 bt = gain_bucket(gain, vwt(d, u));
+                const u32 g = (MAX_BUCKETS * own_b + bt) * d.sections + (u % d.sections);
+                d.gid[u] = g;
+                atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) vwt(d, u));
+                listed = true;
+            }
+        }
+    } else {
+        s64 sum = 0, count = 0;
+        for (u32 i = lane; i < T; i += G) {
+            const u32 b = gc.touched[i];
+            if (b == own_b) continue;
+            if (d.bw[b] < mbw) {
+                sum += (s64) gc.cnt[b];
+                count += 1;
+            }
+        }
+        sum = group_sum<G>(sum, red2);
+        count = group_sum<G>(count, red2);
+        if (active && lane == 0) {
+            const s64 own = (s64) gc.cnt[own_b];
+            if (count == 0) count = 1;
+            const s64 gain = sum / count - own;
+            const s64 wu = vwt(d, u);
+            const s64 lim = d.bw[own_b] - d.lmax;
+            const u32 bt = gain_bucket(gain, wu < lim ? wu : lim);
+            const u32 g = (MAX_BUCKETS * own_b + bt) * d.sections + (u % d.sections) + 1;
+            d.gid[u] = g;
+            atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) wu);
+            listed = true;
+        }
+    }
+    return listed;
+}
+
+// score offset of minibucket g for a vertex of block ub: the weight of the
+// candidates in minibuckets [ub * width, g).  A strong candidate in the last
+// minibucket of its block lands at index (ub + 1) * width (the reference's +1
+// minibucket offset, jet:437), i.e. after its block's whole range.
+__device__ __forceinline__ s64 bucket_prefix(const JetDev &d, u32 g, u32 ub) {
+    return g >= (ub + 1) * d.width ? d.Tseg[ub] : d.Q[g];
+}
+
+// ---------------------------------------------------------------- sweep
+// Each wave walks tiles of 64 consecutive vertices.  LP: cached vertices take
+// their cached decision lane by lane; rebuilt vertices are listed in the
+// wave's LDS slot and evaluated by the wave's 64/G lane groups.  Rebalance:
+// eligible vertices are listed and scored the same way.  Hubs are left to
+// hub_sweep_kernel.  A pending best-partition copy is done on the way.
+template<int G>
+__global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
     extern __shared__ __align__(16) char smem[];
-    __shared__ Best red[JB / WAVE];
-    const u32 groups = blockDim.x / G;
-    const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
-    GroupConn gc = group_conn(smem, d.k, gi);
+    __shared__ u32 s_cnt;
+    __shared__ u64 s_red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl)) return;
+    u32 *region = d.cand + (u64) blockIdx.x * d.cap;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const u32 mode = ctl->mode;
+    const bool save = ctl->save != 0;
+    const double ratio = ctl->ratio;
+    constexpr u32 GPW = WAVE / G;
+    const u32 groups = blockDim.x / G, waves = blockDim.x / WAVE;
+    const u32 wid = threadIdx.x / WAVE, l64 = lane_id();
+    const u32 lane = threadIdx.x % G, gsub = l64 / G;
+    const size_t cgb = conn_group_bytes(d.k);
+    GroupConn gc = group_conn(smem, d.k, threadIdx.x / G);
+    u32 *wlist = (u32 *) (smem + cgb * groups) + wid * WAVE;
+    RebShared *rs = (RebShared *) (smem + cgb * groups + 4ull * WAVE * waves);
     for (u32 i = lane; i < d.k; i += G) gc.cnt[i] = 0;
     if (lane == 0) *gc.tcount = 0;
-    __syncthreads();
-    const u32 len = (u32) d.st[ST_MISS];
-    u64 degs = 0;
-    for (u64 i0 = (u64) blockIdx.x * groups; i0 < len; i0 += (u64) gridDim.x * groups) {
-        const u64 i = i0 + gi;
-        const bool active = i < len;
-        const u32 u = active ? d.miss[i] : 0;
-        if (active && lane == 0) degs += d.xadj[u + 1] - d.xadj[u];
-        if (active) conn_accumulate(d, gc, u, lane, G);
-        __syncthreads();
-        bool cand;
-        lp_eval<G>(d, gc, u, lane, ratio, red, active, cand);
-        __syncthreads();
-        if (active) conn_reset(gc, *gc.tcount, lane, G);
-        __syncthreads();
-        if (active && lane == 0) *gc.tcount = 0;
-        append(cand, u, d.cand, &d.st[ST_CAND]);
+    const bool strong = mode == 2;
+    const s64 mbw = mode ? max_block_weight(d, strong) : 0;
+    if (mode) reb_prologue(d, strong, mbw, rs);
+    wave_sync();
+    u64 nev = 0, degs = 0;
+    const u64 stride = (u64) gridDim.x * waves * WAVE;
+    for (u64 base = ((u64) blockIdx.x * waves + wid) * WAVE; base < d.n; base += stride) {
+        const u64 vv = base + l64;
+        const u32 v = (u32) vv;
+        bool heavy = false, cand = false;
+        if (vv < d.n) {
+            if (save) d.part[v] = d.work[v];
+            if (mode == 0) {
+                const u32 dc = d.dcache[v];
+                if (dc == NULL_PART) {
+                    heavy = degree(d, v) <= d.hub_deg;
+                } else {
+                    d.dest[v] = dc;
+                    if (dc != NO_MOVE && d.lock[v] == 0) {
+                        cand = true;
+                        d.gain1[v] = d.gcache[v];
+                    } else {
+                        d.gain1[v] = GAIN_MIN;
+                        d.lock[v] = 0;
+                    }
+                }
+            } else {
+                heavy = degree(d, v) <= d.hub_deg && reb_eligible(d, strong, rs, v);
+            }
+        }
+        append_local(cand, v, region, &s_cnt);
+        const u64 hm = __ballot(heavy);
+        const u32 nh = (u32) __popcll(hm);
+        if (heavy) wlist[__popcll(hm & lanemask_lt())] = v;
+        wave_sync();
+        for (u32 r0 = 0; r0 < nh; r0 += GPW) {
+            const u32 idx = r0 + gsub;
+            const bool active = idx < nh;
+            const u32 u = active ? wlist[idx] : 0;
+            if (active) {
+                if (lane == 0) {
+                    nev++;
+                    degs += degree(d, u);
+                }
+                conn_accumulate(d, gc, u, lane, G);
+            }
+            wave_sync();
+            const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, active)
+                                          : reb_eval<G>(d, strong, mbw, rs, gc, u, lane, nullptr, nullptr, active);
+            wave_sync();
+            if (active) conn_reset(gc, *gc.tcount, lane, G);
+            wave_sync();
+            if (active && lane == 0) *gc.tcount = 0;
+            wave_sync();
+            append_local(listed, u, region, &s_cnt);
+        }
     }
-    degs = wave_sum(degs);
-    if (lane_id() == 0 && degs) atomicAdd(&d.st[ST_MISS_DEG], (unsigned long long) degs);
+    nev = block_total(nev, s_red);
+    degs = block_total(degs, s_red);
+    if (threadIdx.x == 0) {
+        d.cand_n[blockIdx.x] = s_cnt;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_MISS] = nev;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_MISS_DEG] = degs;
+    }
 }
 
-// K1h: one workgroup per hub (hubs never enter the miss list)
-__global__ void lp_hub_kernel(JetDev d, const u32 *hubs, double ratio) {
+// Per-block prefix of the minibucket totals (the reference's scan over all
+// minibuckets, jet:440-447 / :696-703, restricted to what selection reads:
+// differences inside one block's range).  Workgroup p scans block p's range
+// [p * width, (p + 1) * width) and clears the totals for the next iteration.
+constexpr int BKT_ITEMS = 8;
+__global__ __launch_bounds__(JB) void bucket_kernel(JetDev d) {
+    __shared__ s64 red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl) || ctl->mode == 0) return;
+    const u32 p = blockIdx.x;
+    const u64 base = (u64) p * d.width;
+    s64 carry = 0;
+    for (u32 c0 = 0; c0 < d.width; c0 += JB * BKT_ITEMS) {
+        const u32 first = c0 + threadIdx.x * BKT_ITEMS;
+        s64 v[BKT_ITEMS];
+        s64 sum = 0;
+#pragma unroll
+        for (int j = 0; j < BKT_ITEMS; ++j) {
+            v[j] = first + j < d.width ? d.btot[base + first + j] : 0;
+            sum += v[j];
+        }
+        s64 tot;
+        s64 run = carry + block_exclusive_scan<JB>(sum, red, tot);
+#pragma unroll
+        for (int j = 0; j < BKT_ITEMS; ++j) {
+            if (first + j < d.width) {
+                d.Q[base + first + j] = run;
+                run += v[j];
+                d.R[base + first + j] = run;
+                d.btot[base + first + j] = 0;
+            }
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        d.Tseg[p] = carry;
+        if (p == d.k - 1) d.btot[(u64) d.k * d.width] = 0;  // strong +1 entry of the last block
+    }
+}
+
+// hubs: one workgroup each.  Phase 0 runs before the main sweep and scores
+// rebalance candidates (their weight must be in the minibucket totals before
+// the main sweep's last workgroup scans them); phase 1 runs after it and
+// rebuilds uncached LP vertices (cached ones were taken by the main sweep).
+__global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs, int phase) {
     extern __shared__ __align__(16) char smem[];
     __shared__ Best red[JB / WAVE];
+    __shared__ s64 red2[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl)) return;
+    const u32 mode = ctl->mode;
+    if ((mode == 0) != (phase == 1)) return;
     const u32 u = hubs[blockIdx.x];
-    if (d.dcache[u] != NULL_PART) return;  // cached: done by lp_cached_kernel
     GroupConn gc = group_conn(smem, d.k, 0);
+    RebShared *rs = (RebShared *) (smem + conn_group_bytes(d.k));
+    const bool strong = mode == 2;
+    s64 mbw = 0;
+    if (mode == 0) {
+        if (d.dcache[u] != NULL_PART) return;
+    } else {
+        mbw = max_block_weight(d, strong);
+        reb_prologue(d, strong, mbw, rs);
+        if (!reb_eligible(d, strong, rs, u)) return;
+    }
     for (u32 i = threadIdx.x; i < d.k; i += JB) gc.cnt[i] = 0;
     if (threadIdx.x == 0) *gc.tcount = 0;
     __syncthreads();
     conn_accumulate(d, gc, u, threadIdx.x, JB);
     __syncthreads();
-    bool cand;
-    lp_eval<JB>(d, gc, u, threadIdx.x, ratio, red, true, cand);
-    append(cand, u, d.cand, &d.st[ST_CAND]);
-    if (threadIdx.x == 0) atomicAdd(&d.st[ST_MISS_DEG], (unsigned long long) (d.xadj[u + 1] - d.xadj[u]));
+    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, ctl->ratio, red, true)
+                                  : reb_eval<JB>(d, strong, mbw, rs, gc, u, threadIdx.x, red, red2, true);
+    append(listed, u, d.cand_x, &ctl->st[ST_CAND]);
+    if (threadIdx.x == 0) {
+        atomicAdd(&ctl->st[ST_MISS], 1ull);
+        atomicAdd(&ctl->st[ST_MISS_DEG], (unsigned long long) degree(d, u));
+    }
 }
 
-// K2: afterburner (jet:269-328) over candidates, G lanes each; selected
-// candidates are locked, stamped and appended to the move list
-template<int G>
+// ---------------------------------------------------------------- post: afterburner / classify
+// afterburner (jet:269-328): selected candidates are locked, stamped and
+// appended to the move list
 __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 lane, u32 stride) {
     const s64 ug = d.gain1[u];
     const u32 o = d.work[u], nn = d.dest[u];
@@ -338,73 +669,373 @@ __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 la
 }
 
 template<int G>
-__global__ void afterburner_kernel(JetDev d, int do_hash) {
-    __shared__ s64 red[JB / WAVE];
+__device__ __forceinline__ void afterburner_body(const JetDev &d, u32 stamp, u32 *mregion, u32 *s_cnt, u64 *s_red) {
+    const bool do_hash = d.trace != nullptr;
     const u32 groups = blockDim.x / G;
     const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
-    const u32 len = (u32) d.st[ST_CAND];
+    const u32 *region = d.cand + (u64) blockIdx.x * d.cap;
+    const u32 len = d.cand_n[blockIdx.x];
     u64 gh = 0, degs = 0;
-    for (u64 i0 = (u64) blockIdx.x * groups; i0 < len; i0 += (u64) gridDim.x * groups) {
-        const u64 i = i0 + gi;
+    for (u32 i0 = 0; i0 < len; i0 += groups) {
+        const u32 i = i0 + gi;
         const bool active = i < len;
-        const u32 u = active ? d.cand[i] : 0;
-        const bool hub = active && (d.xadj[u + 1] - d.xadj[u]) > d.hub_deg;
-        if (active && !hub && lane == 0) degs += d.xadj[u + 1] - d.xadj[u];
-        s64 ch = (active && !hub) ? afterburner_change<G>(d, u, lane, G) : 0;
-        ch = group_sum<G>(ch, red);
+        const u32 u = active ? region[i] : 0;
+        const bool hub = active && degree(d, u) > d.hub_deg;
+        s64 ch = (active && !hub) ? afterburner_change(d, u, lane, G) : 0;
+        ch = group_sum<G>(ch, (s64 *) nullptr);
         bool sel = false;
         if (active && !hub && lane == 0) {
+            degs += degree(d, u);
             const s64 ug = d.gain1[u];
             if (do_hash) gh += mix64d(mix64d(u) + (u64) ug);
             if (ug + ch >= 0) {
                 d.lock[u] = 1;
-                d.mark[u] = d.stamp;
+                d.mark[u] = stamp;
                 d.oldp[u] = d.work[u];
                 sel = true;
             }
         }
-        append(sel, u, d.moves, &d.st[ST_MOVES]);
+        append_local(sel, u, mregion, s_cnt);
     }
-    degs = wave_sum(degs);
-    if (lane_id() == 0 && degs) atomicAdd(&d.st[ST_CAND_DEG], (unsigned long long) degs);
-    if (do_hash) {
-        gh = wave_sum(gh);
-        if (lane_id() == 0 && gh) atomicAdd(&d.st[ST_GHASH], (unsigned long long) gh);
+    degs = block_total(degs, s_red);
+    gh = block_total(gh, s_red);
+    if (threadIdx.x == 0) {
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_CAND_DEG] = degs;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_GHASH] = gh;
     }
 }
 
-__global__ void afterburner_hub_kernel(JetDev d, const u32 *hubs, int do_hash) {
+// Rebalance selection (weak jet:718-744, strong jet:472-500) splits the
+// candidates by score(u) = P[gid] - P[block begin] + off(u), off(u) in
+// [0, btot[gid] - w(u)]: not taken if the minibucket starts at or beyond the
+// limit; taken (weak) if the whole minibucket ends below the limit; the rest
+// (and, for strong, every possibly taken candidate, whose exact position feeds
+// adjust_scores) goes to the resolve list.  Both tests depend on the
+// minibucket only, so a minibucket is either wholly in the resolve list or not
+// at all, and offsets computed over the list are offsets in the minibucket.
+__device__ __forceinline__ void classify_one(const JetDev &d, bool strong, u32 stamp, bool valid, u32 u,
+                                             u32 *mregion, u32 *s_cnt) {
+    JetCtl *ctl = d.ctl;
+    {
+        bool sure = false, boundary = false;
+        u64 key = 0;
+        if (valid) {
+            const u32 g = d.gid[u], ub = d.work[u];
+            const s64 limit = d.bw[ub] - d.lmax;
+            if (bucket_prefix(d, g, ub) < limit) {
+                if (!strong && d.R[g] < limit) {
+                    sure = true;
+                    d.mark[u] = stamp;
+                    d.oldp[u] = ub;
+                } else {
+                    boundary = true;
+                    key = ((u64) g << 32) | u;
+                }
+            }
+        }
+        append_local(sure, u, mregion, s_cnt);
+        append64(boundary, key, d.rlist, &ctl->st[ST_RLIST]);
+    }
+}
+
+// this workgroup's candidate region, then its share of the extra list
+__device__ __forceinline__ void classify_body(const JetDev &d, bool strong, u32 stamp, u32 *mregion, u32 *s_cnt) {
+    const u32 *region = d.cand + (u64) blockIdx.x * d.cap;
+    const u32 len = d.cand_n[blockIdx.x];
+    for (u32 i0 = 0; i0 < len; i0 += blockDim.x) {
+        const u32 i = i0 + threadIdx.x;
+        classify_one(d, strong, stamp, i < len, i < len ? region[i] : 0u, mregion, s_cnt);
+    }
+    const u32 lx = (u32) d.ctl->st[ST_CAND];
+    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < lx; i0 += (u64) gridDim.x * blockDim.x) {
+        const u64 i = i0 + threadIdx.x;
+        classify_one(d, strong, stamp, i < lx, i < lx ? d.cand_x[i] : 0u, mregion, s_cnt);
+    }
+}
+
+template<int G>
+__global__ __launch_bounds__(JB) void post_kernel(JetDev d) {
+    __shared__ u32 s_cnt;
+    __shared__ u64 s_red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl)) return;
+    const u32 mode = ctl->mode, stamp = ctl->stamp;
+    u32 *mregion = d.moves + (u64) blockIdx.x * d.cap_mv;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    if (mode == 0) {
+        afterburner_body<G>(d, stamp, mregion, &s_cnt, s_red);
+    } else {
+        classify_body(d, mode == 2, stamp, mregion, &s_cnt);
+        if (threadIdx.x == 0) {
+            d.pstat[(u64) blockIdx.x * PT_COUNT + PT_CAND_DEG] = 0;
+            d.pstat[(u64) blockIdx.x * PT_COUNT + PT_GHASH] = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) d.moves_n[blockIdx.x] = s_cnt;
+}
+
+__global__ __launch_bounds__(JB) void hub_post_kernel(JetDev d, const u32 *hubs) {
     __shared__ s64 red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl) || ctl->mode != 0) return;
+    const u32 stamp = ctl->stamp;
     const u32 u = hubs[blockIdx.x];
     const s64 ug = d.gain1[u];
     if (ug == GAIN_MIN) return;  // not a candidate
-    s64 ch = afterburner_change<JB>(d, u, threadIdx.x, JB);
+    s64 ch = afterburner_change(d, u, threadIdx.x, JB);
     ch = group_sum<JB>(ch, red);
     bool sel = false;
     if (threadIdx.x == 0) {
-        atomicAdd(&d.st[ST_CAND_DEG], (unsigned long long) (d.xadj[u + 1] - d.xadj[u]));
-        if (do_hash) atomicAdd(&d.st[ST_GHASH], (unsigned long long) mix64d(mix64d(u) + (u64) ug));
+        atomicAdd(&ctl->st[ST_CAND_DEG], (unsigned long long) degree(d, u));
+        if (d.trace) atomicAdd(&ctl->st[ST_GHASH], (unsigned long long) mix64d(mix64d(u) + (u64) ug));
         if (ug + ch >= 0) {
             d.lock[u] = 1;
-            d.mark[u] = d.stamp;
+            d.mark[u] = stamp;
             d.oldp[u] = d.work[u];
             sel = true;
         }
     }
-    append(sel, u, d.moves, &d.st[ST_MOVES]);
+    append(sel, u, d.moves_x, &ctl->st[ST_MOVES]);
 }
 
-// ---------------------------------------------------------------- apply moves
-// K3: exact cut change, cache invalidation (u and N(u)), partition and block
-// weight update.  Non-moved neighbours are never written, moved neighbours'
-// old block comes from oldp, so the concurrent writes to work[] are safe.
-template<int G>
-__device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 stride) {
+// ---------------------------------------------------------------- resolve
+// the chunk containing position pos: the last id in [0, k] with es[id] <= pos
+// (es = evict_start, non-decreasing from es[0] = 0; the reference walks it
+// linearly, jet:564-566)
+__device__ __forceinline__ s64 chunk_index(const s64 *es, u32 k, s64 pos) {
+    u32 lo = 0, hi = k + 1;
+    while (lo < hi) {
+        const u32 mid = (lo + hi) >> 1;
+        if (es[mid] <= pos) lo = mid + 1;
+        else hi = mid;
+    }
+    return (s64) lo - 1;
+}
+
+// destination chunk of an evicted vertex at position pos (adjust_scores,
+// jet:563-575, first pass): id < k with room -> that block; id >= k -> stays;
+// otherwise the vertex must take an overflow position (returns NO_MOVE)
+__device__ __forceinline__ u32 chunk_of(const s64 *es, u32 k, s64 pos, s64 wu, u32 ub) {
+    const s64 id = chunk_index(es, k, pos);
+    if (id >= 0 && id < (s64) k && wu / 2 <= es[id + 1] - pos) return (u32) id;
+    if (id < (s64) k) return NO_MOVE;
+    return ub;
+}
+
+// the serial adjust_scores loop of one overflowing vertex (jet:563-585)
+__device__ __forceinline__ u32 overflow_dest(const s64 *es, u32 k, s64 pos, s64 wu, u32 ub, s64 &counter) {
+    s64 id = 0;
+    while (id < (s64) k) {
+        id = chunk_index(es, k, pos);
+        if (id >= 0 && id < (s64) k && wu / 2 <= es[id + 1] - pos) return (u32) id;
+        if (id < (s64) k) {
+            pos = counter;
+            counter += wu;
+        }
+    }
+    return ub;
+}
+
+// cookie cutter (jet:508-552), by the first wave of the workgroup; on entry
+// evict_adjust holds the filter's values, on exit the eviction position of
+// each block's first candidate, so that position = score + evict_adjust[block]
+__device__ void cookie_cutter(const JetDev &d, s64 mbw) {
+    const u32 lane = threadIdx.x;
+    s64 carry = 0, carry2 = 0;
+    for (u32 b0 = 0; b0 < d.k; b0 += WAVE) {
+        const u32 p = b0 + lane;
+        s64 add = 0, cap = 0;
+        if (p < d.k) {
+            add = d.evict_adjust[p];
+            if (add == 0) add = d.Tseg[p];
+            cap = mbw > d.bw[p] ? mbw - d.bw[p] : 0;
+        }
+        const s64 inc = wave_inclusive_scan(add), inc2 = wave_inclusive_scan(cap);
+        if (p < d.k) {
+            d.evict_adjust[p] = carry + inc - add;
+            d.evict_start[p + 1] = carry2 + inc2;
+        }
+        carry += __shfl(inc, WAVE - 1, WAVE);
+        carry2 += __shfl(inc2, WAVE - 1, WAVE);
+    }
+    if (lane == 0) {
+        *d.counter = carry;
+        d.evict_start[0] = 0;
+    }
+}
+
+// One workgroup sorts the resolve list by (minibucket, id) in LDS, derives each
+// entry's offset inside its minibucket and selects; strong rebalancing then
+// runs the cookie cutter and adjust_scores.  Lists longer than CAP, or more
+// than OVF_CAP overflowing vertices, stall the loop for the host's global path
+// (nothing has been appended to the move list at that point).
+template<u32 CAP>
+__global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
+    __shared__ u64 skey[CAP];
+    __shared__ s64 soff[CAP];
+    __shared__ u8 ssel[CAP];
+    __shared__ s64 red[RB / WAVE];
+    __shared__ u32 ocount;
+    __shared__ u32 olist[OVF_CAP];
+    __shared__ s64 s_es[ES_CAP];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl) || ctl->mode == 0) return;
+    const bool strong = ctl->mode == 2;
+    const u32 stamp = ctl->stamp;
+    const u32 L = (u32) ctl->st[ST_RLIST];
+    if (L == 0) return;
+    if (L > CAP || L > d.rcap) {
+        if (threadIdx.x == 0) ctl->stalled = 1;
+        return;
+    }
+    u32 P2 = 1;
+    while (P2 < L) P2 <<= 1;
+    if (threadIdx.x == 0) ocount = 0;
+    if (strong)
+        for (u32 b = threadIdx.x; b < d.k; b += RB) d.evict_adjust[b] = 0;
+    for (u32 i = threadIdx.x; i < P2; i += RB) skey[i] = i < L ? d.rlist[i] : ~0ull;
+    __syncthreads();
+    // bitonic sort, ascending
+    for (u32 kk = 2; kk <= P2; kk <<= 1) {
+        for (u32 j = kk >> 1; j > 0; j >>= 1) {
+            for (u32 i = threadIdx.x; i < P2; i += RB) {
+                const u32 p = i ^ j;
+                if (p > i) {
+                    const u64 a = skey[i], b = skey[p];
+                    if (((i & kk) == 0) ? (a > b) : (a < b)) {
+                        skey[i] = b;
+                        skey[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // exclusive prefix of the weights in sorted order
+    s64 carry = 0;
+    for (u32 i0 = 0; i0 < L; i0 += RB) {
+        const u32 i = i0 + threadIdx.x;
+        const s64 w = i < L ? vwt(d, (u32) skey[i]) : 0;
+        s64 tot;
+        const s64 e = block_exclusive_scan<RB>(w, red, tot);
+        if (i < L) soff[i] = carry + e;
+        carry += tot;
+    }
+    __syncthreads();
+    // offset inside the minibucket = prefix - prefix at the minibucket's first entry
+    s64 offv[CAP / RB];
+#pragma unroll
+    for (u32 q = 0; q < CAP / RB; ++q) {
+        const u32 i = q * RB + threadIdx.x;
+        offv[q] = 0;
+        if (i < L) {
+            const u64 head = skey[i] & 0xFFFFFFFF00000000ull;
+            u32 lo = 0, hi = i;
+            while (lo < hi) {
+                const u32 mid = (lo + hi) >> 1;
+                if (skey[mid] < head) lo = mid + 1;
+                else hi = mid;
+            }
+            offv[q] = soff[i] - soff[lo];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 q = 0; q < CAP / RB; ++q) {
+        const u32 i = q * RB + threadIdx.x;
+        if (i < L) soff[i] = offv[q];
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < L; i += RB) {
+        const u32 u = (u32) skey[i];
+        const u32 g = (u32) (skey[i] >> 32);
+        const u32 ub = d.work[u];
+        const s64 score = bucket_prefix(d, g, ub) + soff[i];
+        const s64 limit = d.bw[ub] - d.lmax;
+        const bool take = score < limit;
+        ssel[i] = take;
+        if (take && strong && score + vwt(d, u) >= limit) d.evict_adjust[ub] = score + vwt(d, u);
+        if (take && !strong) {
+            d.mark[u] = stamp;
+            d.oldp[u] = ub;
+        }
+    }
+    __syncthreads();
+    if (strong) {
+        if (threadIdx.x < WAVE) cookie_cutter(d, max_block_weight(d, true));
+        __syncthreads();
+        const bool es_lds = d.k + 1 <= ES_CAP;
+        if (es_lds)
+            for (u32 i = threadIdx.x; i <= d.k; i += RB) s_es[i] = d.evict_start[i];
+        __syncthreads();
+        const s64 *es = es_lds ? s_es : d.evict_start;
+        // adjust_scores, first pass; overflowing vertices are resolved below
+        for (u32 i = threadIdx.x; i < L; i += RB) {
+            if (!ssel[i]) continue;
+            const u32 u = (u32) skey[i];
+            const u32 g = (u32) (skey[i] >> 32);
+            const u32 ub = d.work[u];
+            const s64 pos = bucket_prefix(d, g, ub) + soff[i] + d.evict_adjust[ub];
+            const u32 dst = chunk_of(es, d.k, pos, vwt(d, u), ub);
+            d.mark[u] = stamp;
+            d.oldp[u] = ub;
+            if (dst == NO_MOVE) {
+                const u32 q = atomicAdd(&ocount, 1u);
+                if (q < OVF_CAP) olist[q] = i;
+                ssel[i] = 2;
+            } else {
+                d.dest[u] = dst;
+            }
+        }
+        __syncthreads();
+        const u32 oc = ocount;
+        if (oc > OVF_CAP) {
+            if (threadIdx.x == 0) ctl->stalled = 1;
+            return;
+        }
+        if (threadIdx.x == 0 && oc) {
+            // serial overflow in vertex id order (insertion sort of the entries by id)
+            for (u32 a = 1; a < oc; ++a) {
+                const u32 x = olist[a];
+                u32 b = a;
+                while (b > 0 && (u32) skey[olist[b - 1]] > (u32) skey[x]) {
+                    olist[b] = olist[b - 1];
+                    --b;
+                }
+                olist[b] = x;
+            }
+            s64 counter = *d.counter;
+            for (u32 q = 0; q < oc; ++q) {
+                const u32 i = olist[q];
+                const u32 u = (u32) skey[i];
+                const u32 g = (u32) (skey[i] >> 32);
+                const u32 ub = d.work[u];
+                const s64 pos = bucket_prefix(d, g, ub) + soff[i] + d.evict_adjust[ub];
+                d.dest[u] = overflow_dest(es, d.k, pos, vwt(d, u), ub, counter);
+            }
+            *d.counter = counter;
+        }
+        __syncthreads();
+    }
+    for (u32 i0 = 0; i0 < L; i0 += RB) {
+        const u32 i = i0 + threadIdx.x;
+        const bool take = i < L && ssel[i] != 0;
+        append(take, take ? (u32) skey[i] : 0u, d.moves_x, &ctl->st[ST_MOVES]);
+    }
+}
+
+// ---------------------------------------------------------------- apply moves and control
+// exact cut change, cache invalidation (u and N(u)), partition and block
+// weight update.  Non-moved neighbours are never written and moved
+// neighbours' old block comes from oldp, so concurrent writes to work[] are safe.
+__device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 stride, u32 stamp) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
     s64 delta = 0;
     for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
         const u32 v = d.adj[j];
-        const bool mv = d.mark[v] == d.stamp;
+        const bool mv = d.mark[v] == stamp;
         const u32 ov = mv ? d.oldp[v] : d.work[v];
         const u32 nv = mv ? d.dest[v] : ov;
         const s64 x = ewt(d, j) * ((s64) (nu != nv) - (s64) (ou != ov));
@@ -414,251 +1045,276 @@ __device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 
     return delta;
 }
 
-template<int G>
-__global__ void apply_kernel(JetDev d, int do_hash) {
-    __shared__ s64 red[JB / WAVE];
-    const u32 groups = blockDim.x / G;
-    const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
-    const u32 len = (u32) d.st[ST_MOVES];
-    s64 acc = 0;
-    u64 mh = 0, degs = 0;
-    for (u64 i0 = (u64) blockIdx.x * groups; i0 < len; i0 += (u64) gridDim.x * groups) {
-        const u64 i = i0 + gi;
-        const bool active = i < len;
-        const u32 u = active ? d.moves[i] : 0;
-        const bool hub = active && (d.xadj[u + 1] - d.xadj[u]) > d.hub_deg;
-        if (active && !hub) {
-            acc += move_delta<G>(d, u, lane, G);
-            if (lane == 0) {
-                degs += d.xadj[u + 1] - d.xadj[u];
-                const u32 ou = d.oldp[u], nu = d.dest[u];
-                const s64 w = vwt(d, u);
-                d.dcache[u] = NULL_PART;
-                d.work[u] = nu;
-                atomicAdd((unsigned long long *) &d.bw[ou], (unsigned long long) (-w));
-                atomicAdd((unsigned long long *) &d.bw[nu], (unsigned long long) w);
-                if (do_hash) mh += mix64d(((u64) u << 32) | nu);
-            }
+__device__ __forceinline__ void move_commit(const JetDev &d, u32 u) {
+    const u32 ou = d.oldp[u], nu = d.dest[u];
+    const s64 w = vwt(d, u);
+    d.dcache[u] = NULL_PART;
+    d.work[u] = nu;
+    atomicAdd((unsigned long long *) &d.bw[ou], (unsigned long long) (-w));
+    atomicAdd((unsigned long long *) &d.bw[nu], (unsigned long long) w);
+}
+
+// algorithmic bytes of one iteration's launches (w = 2 when edge weights are
+// stored, else 1; E(X) = summed degree of the vertex set X):
+//   LP sweep: 25n (cache, lock, dest, gain) + 12 per rebuilt vertex + 4w E +
+//     4 per distinct neighbour block read (capped at 4n);
+//   rebalance sweep: (8 + 4 if vertex weights) n + 4w E(scored) + min(4E, 4n)
+//     + 8 per candidate;
+//   buckets: 32 t (read and clear the totals, write two prefixes);
+//   afterburner: 16|C| + 4w E(C) + min(16 E(C), 16n);
+//   classify: (20 + 4 if vertex weights) per candidate;
+//   resolve: 32 per resolve-list entry;
+//   apply: 16|M| + 4w E(M) + min(16 E(M), 16n);
+//   a pending best-partition copy adds 8n to the sweep.
+__device__ void account(const JetDev &d, JetCtl *ctl, const unsigned long long *rep, u32 mode, bool saved) {
+    const double w = d.ew ? 2.0 : 1.0, n = (double) d.n, wv = d.vw ? 4.0 : 0.0;
+    const double nm = (double) rep[ST_MISS], em = (double) rep[ST_MISS_DEG];
+    const double nc = (double) rep[ST_CAND], ec = (double) rep[ST_CAND_DEG];
+    const double nmv = (double) rep[ST_MOVES], emv = (double) rep[ST_MOVE_DEG];
+    double sweep = saved ? 8.0 * n : 0.0, buckets = 0.0, post, resolve = 0.0;
+    if (mode == 0) {
+        sweep += 25.0 * n + 12.0 * nm + 4.0 * w * em + fmin(4.0 * em, 4.0 * n);
+        post = 16.0 * nc + 4.0 * w * ec + fmin(16.0 * ec, 16.0 * n);
+    } else {
+        sweep += (8.0 + wv) * n + 4.0 * w * em + fmin(4.0 * em, 4.0 * n) + 8.0 * nc;
+        buckets = 32.0 * (double) d.t;
+        post = (20.0 + wv) * nc;
+        resolve = 32.0 * (double) rep[ST_RLIST];
+    }
+    ctl->bytes[0] += sweep;
+    ctl->bytes[1] += buckets;
+    ctl->bytes[2] += post;
+    ctl->bytes[3] += resolve;
+    ctl->bytes[4] += 16.0 * nmv + 4.0 * w * emv + fmin(16.0 * emv, 16.0 * n);
+}
+
+// One step of jet_refine's loop (jet:960-1030) after an iteration: counters,
+// cut and block weight, best-partition tracking, ratio and mode of the next
+// iteration.  Runs on one thread of the control launch.
+__device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot) {
+    JetCtl cl = *d.ctl;  // one batch of loads; written back at the end
+    JetCtl *ctl = &cl;
+    unsigned long long rep[ST_COUNT];
+#pragma unroll
+    for (u32 s = 0; s < ST_COUNT; ++s) {
+        rep[s] = ctl->st[s];
+        ctl->st[s] = 0;
+    }
+    rep[ST_MISS] += tot[PT_MISS];
+    rep[ST_MISS_DEG] += tot[PT_MISS_DEG];
+    rep[ST_CAND_DEG] += tot[PT_CAND_DEG];
+    rep[ST_GHASH] += tot[PT_GHASH];
+    rep[ST_DELTA] += tot[PT_DELTA];
+    rep[ST_MOVE_DEG] += tot[PT_MOVE_DEG];
+    rep[ST_MHASH] += tot[PT_MHASH];
+    rep[ST_CAND] += tot[PT_COUNT];
+    rep[ST_MOVES] += tot[PT_COUNT + 1];
+    const u32 mode = ctl->mode;
+    const u32 n_moves = (u32) rep[ST_MOVES];
+    account(d, ctl, rep, mode, ctl->save != 0);
+    u32 bal = ctl->bal, it = ctl->it, ri = ctl->ri;
+    bool counted = true, end_ratio = false;
+    if (mode == 0) {
+        bal = 0;
+        if (n_moves == 0) end_ratio = true;
+    } else if (mode == 1) {
+        if (n_moves == 0) {
+            bal = N_MAX_WEAK_ITERATIONS;
+            counted = false;
+        } else {
+            bal++;
+        }
+    } else {
+        if (n_moves == 0) end_ratio = true;
+        else bal++;
+    }
+    u32 save = 0;
+    if (counted && !end_ratio) {
+        ctl->iters++;
+        const s64 cur_cut = ctl->cur_cut + (s64) rep[ST_DELTA] / 2;
+        const s64 cur_maxw = maxw;
+        ctl->cur_cut = cur_cut;
+        ctl->cur_maxw = cur_maxw;
+        if (d.trace) {
+            heipa_jet_trace &t = d.trace[ctl->n_trace % TRACE_RING];
+            t.ratio_idx = ri;
+            t.iteration = it;
+            t.mode = mode;
+            t.n_cand = (u32) rep[ST_CAND];
+            t.n_moves = n_moves;
+            t.pad = 0;
+            t.cut = cur_cut;
+            t.maxw = cur_maxw;
+            t.moves_hash = rep[ST_MHASH];
+            t.gain_hash = mode == 0 ? rep[ST_GHASH] : 0;
+            ctl->n_trace++;
+        }
+        const s64 best_cut = ctl->best_cut, best_maxw = ctl->best_maxw;
+        if (best_maxw > d.lmax && cur_maxw < best_maxw) {
+            save = 1;
+            ctl->best_cut = cur_cut;
+            ctl->best_maxw = cur_maxw;
+            it = 0;
+        } else if (cur_cut < best_cut && (cur_maxw <= d.lmax || cur_maxw <= best_maxw)) {
+            if ((double) cur_cut < PHI * (double) best_cut) it = 0;
+            save = 1;
+            ctl->best_cut = cur_cut;
+            ctl->best_maxw = cur_maxw;
         }
     }
-    acc = block_sum<JB>(acc, red);
-    if (threadIdx.x == 0 && acc) atomicAdd(&d.st[ST_DELTA], (unsigned long long) acc);
-    degs = wave_sum(degs);
-    if (lane_id() == 0 && degs) atomicAdd(&d.st[ST_MOVE_DEG], (unsigned long long) degs);
-    if (do_hash) {
-        mh = wave_sum(mh);
-        if (lane_id() == 0 && mh) atomicAdd(&d.st[ST_MHASH], (unsigned long long) mh);
+    ctl->save = save;
+    if (end_ratio || it >= N_MAX_ITERATIONS) {
+        ri++;
+        bal = 0;
+        it = 0;
     }
+    if (ri >= d.n_ratios) {
+        ctl->active = 0;
+    } else {
+        it += 1;
+        ctl->mode = ctl->cur_maxw <= d.lmax ? 0u : (bal < N_MAX_WEAK_ITERATIONS ? 1u : 2u);
+        ctl->ratio = d.ratios[ri];
+    }
+    ctl->ri = ri;
+    ctl->it = it;
+    ctl->bal = bal;
+    ctl->stamp++;
+    *d.ctl = cl;
 }
 
-__global__ void apply_hub_kernel(JetDev d, const u32 *hubs, int do_hash) {
+__global__ __launch_bounds__(JB) void apply_hub_kernel(JetDev d, const u32 *hubs) {
     __shared__ s64 red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl)) return;
+    const u32 stamp = ctl->stamp;
     const u32 u = hubs[blockIdx.x];
-    if (d.mark[u] != d.stamp) return;
-    s64 acc = move_delta<JB>(d, u, threadIdx.x, JB);
+    if (d.mark[u] != stamp) return;
+    s64 acc = move_delta(d, u, threadIdx.x, JB, stamp);
     acc = block_sum<JB>(acc, red);
     if (threadIdx.x == 0) {
-        if (acc) atomicAdd(&d.st[ST_DELTA], (unsigned long long) acc);
-        atomicAdd(&d.st[ST_MOVE_DEG], (unsigned long long) (d.xadj[u + 1] - d.xadj[u]));
-        const u32 ou = d.oldp[u], nu = d.dest[u];
-        const s64 w = vwt(d, u);
-        d.dcache[u] = NULL_PART;
-        d.work[u] = nu;
-        atomicAdd((unsigned long long *) &d.bw[ou], (unsigned long long) (-w));
-        atomicAdd((unsigned long long *) &d.bw[nu], (unsigned long long) w);
-        if (do_hash) atomicAdd(&d.st[ST_MHASH], (unsigned long long) mix64d(((u64) u << 32) | nu));
+        if (acc) atomicAdd(&ctl->st[ST_DELTA], (unsigned long long) acc);
+        atomicAdd(&ctl->st[ST_MOVE_DEG], (unsigned long long) degree(d, u));
+        move_commit(d, u);
+        if (d.trace) atomicAdd(&ctl->st[ST_MHASH], (unsigned long long) mix64d(((u64) u << 32) | d.dest[u]));
     }
 }
 
-// K4: max block weight into the report, copy of the counters
-__global__ void finish_kernel(JetDev d, unsigned long long *report) {
-    __shared__ s64 red[1024 / WAVE];
+// G lanes per move: this workgroup's move region, then its share of the
+// extra list
+template<int G>
+__device__ __forceinline__ void apply_one(const JetDev &d, u32 u, bool active, u32 lane, u32 stamp, bool do_hash,
+                                          s64 &acc, u64 &degs, u64 &mh) {
+    const bool hub = active && degree(d, u) > d.hub_deg;
+    if (active && !hub) {
+        acc += move_delta(d, u, lane, G, stamp);
+        if (lane == 0) {
+            degs += degree(d, u);
+            move_commit(d, u);
+            if (do_hash) mh += mix64d(((u64) u << 32) | d.dest[u]);
+        }
+    }
+}
+
+template<int G>
+__global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
+    __shared__ u64 s_red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl)) return;
+    const u32 stamp = ctl->stamp;
+    const bool do_hash = d.trace != nullptr;
+    const u32 groups = blockDim.x / G;
+    const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
+    const u32 *region = d.moves + (u64) blockIdx.x * d.cap_mv;
+    const u32 len = d.moves_n[blockIdx.x];
+    s64 acc = 0;
+    u64 mh = 0, degs = 0;
+    for (u32 i0 = 0; i0 < len; i0 += groups) {
+        const u32 i = i0 + gi;
+        apply_one<G>(d, i < len ? region[i] : 0u, i < len, lane, stamp, do_hash, acc, degs, mh);
+    }
+    const u32 lx = (u32) ctl->st[ST_MOVES];
+    for (u64 i0 = (u64) blockIdx.x * groups; i0 < lx; i0 += (u64) gridDim.x * groups) {
+        const u64 i = i0 + gi;
+        apply_one<G>(d, i < lx ? d.moves_x[i] : 0u, i < lx, lane, stamp, do_hash, acc, degs, mh);
+    }
+    const u64 a = block_total((u64) acc, s_red);
+    degs = block_total(degs, s_red);
+    mh = block_total(mh, s_red);
+    if (threadIdx.x == 0) {
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_DELTA] = a;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_MOVE_DEG] = degs;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_MHASH] = mh;
+    }
+}
+
+// one workgroup: the largest block weight and the sums of the per-workgroup
+// partials, then control_step
+__global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
+    __shared__ s64 red[JB / WAVE];
+    __shared__ u64 s_red[JB / WAVE];
+    if (!live(d.ctl)) return;
     s64 mx = INT64_MIN;
-    for (u32 b = threadIdx.x; b < d.k; b += blockDim.x) mx = d.bw[b] > mx ? d.bw[b] : mx;
+    for (u32 b = threadIdx.x; b < d.k; b += JB) mx = d.bw[b] > mx ? d.bw[b] : mx;
     mx = wave_max(mx);
     if (lane_id() == 0) red[threadIdx.x / WAVE] = mx;
+    u64 t[PT_COUNT + 2];
+#pragma unroll
+    for (u32 j = 0; j < PT_COUNT + 2; ++j) t[j] = 0;
+    for (u32 r = threadIdx.x; r < d.nreg; r += JB) {
+#pragma unroll
+        for (u32 j = 0; j < PT_COUNT; ++j) t[j] += d.pstat[(u64) r * PT_COUNT + j];
+        t[PT_COUNT] += d.cand_n[r];
+        t[PT_COUNT + 1] += d.moves_n[r];
+    }
+#pragma unroll
+    for (u32 j = 0; j < PT_COUNT + 2; ++j) t[j] = block_total(t[j], s_red);
     __syncthreads();
     if (threadIdx.x == 0) {
         s64 m = INT64_MIN;
-        for (u32 w = 0; w < blockDim.x / WAVE; ++w) m = red[w] > m ? red[w] : m;
-        d.st[ST_MAXW] = (unsigned long long) m;
-        for (u32 s = 0; s < ST_COUNT; ++s) report[s] = d.st[s];
+        for (u32 w = 0; w < JB / WAVE; ++w) m = red[w] > m ? red[w] : m;
+        control_step(d, m, t);
     }
 }
 
-// ---------------------------------------------------------------- rebalancing
-struct RebDev {
-    u32 sections, width, t;
-    s64 max_b_w, opt;
-    u32 *under;         // underloaded blocks (weak), ascending
-    u32 *n_under;
-    u32 *gid;           // minibucket per vertex
-    u32 *flag;          // candidate flag per vertex
-    s64 *btot;          // bucket totals, t + 1
-    s64 *max_vwgt;      // strong: max capacity; then the overflow counter
-    s64 *evict_adjust;  // k
-    s64 *evict_start;   // k + 1
+struct RatioTable {
+    double r[MAX_RATIOS];
+    u32 n;
 };
 
-__global__ void weak_prepare_kernel(JetDev d, RebDev r) {
-    // underloaded blocks in ascending order (jet:614-632), one block of threads
-    __shared__ u32 lds[1024 / WAVE];
-    u32 carry = 0;
-    for (u32 b0 = 0; b0 < d.k; b0 += blockDim.x) {
-        const u32 b = b0 + threadIdx.x;
-        const u32 f = b < d.k && d.bw[b] < r.max_b_w;
-        u32 tot;
-        const u32 e = block_exclusive_scan<1024>(f, lds, tot);
-        if (f) r.under[carry + e] = b;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) *r.n_under = carry;
+__global__ void ctl_init_kernel(JetDev d, RatioTable rt, s64 cut, s64 maxw) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    JetCtl *ctl = d.ctl;
+#pragma unroll
+    for (u32 s = 0; s < ST_COUNT; ++s) ctl->st[s] = 0;
+    ctl->cur_cut = ctl->best_cut = cut;
+    ctl->cur_maxw = ctl->best_maxw = maxw;
+    ctl->ratio = rt.r[0];
+#pragma unroll
+    for (u32 i = 0; i < 5; ++i) ctl->bytes[i] = 0.0;
+    ctl->ri = 0;
+    ctl->it = 1;
+    ctl->bal = 0;
+    ctl->mode = maxw <= d.lmax ? 0u : 1u;
+    ctl->active = 1;
+    ctl->stalled = 0;
+    ctl->save = 0;
+    ctl->iters = 0;
+    ctl->n_trace = 0;
+    ctl->stamp = 1;
+    double *ratios = (double *) d.ratios;
+#pragma unroll
+    for (u32 i = 0; i < MAX_RATIOS; ++i) ratios[i] = rt.r[i];
 }
 
-__global__ void strong_prepare_kernel(JetDev d, RebDev r) {
-    // max capacity of blocks below max_b_w (jet:374-388); -1 if none
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        s64 mx = -1;
-        for (u32 b = 0; b < d.k; ++b)
-            if (d.bw[b] < r.max_b_w) {
-                const s64 cap = r.max_b_w - d.bw[b];
-                if (cap > mx) mx = cap;
-            }
-        *r.max_vwgt = mx;
-    }
+// ---------------------------------------------------------------- global resolve path
+// For resolve lists longer than the LDS capacity: a device-wide radix sort by
+// (minibucket, id), offsets from a scan of the sorted weights, then the same
+// selection, cookie cutter and adjust_scores as resolve_kernel.
+__global__ void fb_vals_kernel(const u64 *keys, u32 n, u32 *vals) {
+    for (u64 i = (u64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64) gridDim.x * blockDim.x)
+        vals[i] = (u32) keys[i];
 }
 
-// weak best block (jet:658-694) / strong score (jet:399-435): G lanes per vertex
-template<int G, bool STRONG>
-__device__ __forceinline__ void reb_eval(const JetDev &d, const RebDev &r, const GroupConn &gc, u32 u, u32 lane,
-                                         Best *red, s64 *red2, bool active) {
-    const u32 T = active ? *gc.tcount : 0;
-    const u32 own_b = active ? d.work[u] : 0;
-    if constexpr (!STRONG) {
-        Best best{0, NO_MOVE};
-        for (u32 i = lane; i < T; i += G) {
-            const u32 b = gc.touched[i];
-            if (d.bw[b] >= r.max_b_w) continue;
-            Best x{(s64) gc.cnt[b], b};
-            if (best_better(x, best)) best = x;
-        }
-        best = best_reduce<G>(best, red);
-        if (active && lane == 0) {
-            const s64 own = (s64) gc.cnt[own_b];
-            u32 dst = own_b;
-            s64 gain;
-            if (best.b != NO_MOVE && best.c > 0) {
-                dst = best.b;
-                gain = best.c - own;
-            } else {
-                const u32 nu = *r.n_under;
-                dst = nu ? r.under[u % nu] : own_b;
-                gain = -own;
-            }
-            d.dest[u] = dst;
-            if (dst != own_b) {
-                const u32 bt = gain_bucket(gain, vwt(d, u));
-                r.gid[u] = (MAX_BUCKETS * own_b + bt) * r.sections + (u % r.sections);
-                r.flag[u] = 1;
-                atomicAdd((unsigned long long *) &r.btot[r.gid[u]], (unsigned long long) vwt(d, u));
-            }
-        }
-    } else {
-        s64 sum = 0, count = 0;
-        for (u32 i = lane; i < T; i += G) {
-            const u32 b = gc.touched[i];
-            if (b == own_b) continue;
-            if (d.bw[b] < r.max_b_w) {
-                sum += (s64) gc.cnt[b];
-                count += 1;
-            }
-        }
-        sum = group_sum<G>(sum, red2);
-        count = group_sum<G>(count, red2);
-        if (active && lane == 0) {
-            const s64 own = (s64) gc.cnt[own_b];
-            if (count == 0) count = 1;
-            const s64 gain = sum / count - own;
-            const s64 wu = vwt(d, u);
-            const s64 lim = d.bw[own_b] - d.lmax;
-            const u32 bt = gain_bucket(gain, wu < lim ? wu : lim);
-            r.gid[u] = (MAX_BUCKETS * own_b + bt) * r.sections + (u % r.sections) + 1;
-            r.flag[u] = 1;
-            atomicAdd((unsigned long long *) &r.btot[r.gid[u]], (unsigned long long) wu);
-        }
-    }
-}
-
-template<bool STRONG>
-__device__ __forceinline__ bool reb_eligible(const JetDev &d, const RebDev &r, u32 u) {
-    const u32 ub = d.work[u];
-    const s64 bwu = d.bw[ub];
-    const s64 wu = vwt(d, u);
-    if constexpr (!STRONG) {
-        return bwu > d.lmax && (double) wu < 1.5 * (double) (bwu - r.opt);
-    } else {
-        const s64 mv = *r.max_vwgt;
-        return mv >= 0 && bwu > d.lmax && wu <= 2 * mv && wu < 2 * (bwu - r.opt);
-    }
-}
-
-template<int G, bool STRONG>
-__global__ void reb_score_kernel(JetDev d, RebDev r) {
-    extern __shared__ __align__(16) char smem[];
-    __shared__ Best red[JB / WAVE];
-    __shared__ s64 red2[JB / WAVE];
-    const u32 groups = blockDim.x / G;
-    const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
-    GroupConn gc = group_conn(smem, d.k, gi);
-    for (u32 i = lane; i < d.k; i += G) gc.cnt[i] = 0;
-    if (lane == 0) *gc.tcount = 0;
-    __syncthreads();
-    for (u64 i0 = (u64) blockIdx.x * groups; i0 < d.n; i0 += (u64) gridDim.x * groups) {
-        const u64 uu = i0 + gi;
-        const u32 u = (u32) uu;
-        bool active = uu < d.n;
-        if (active) active = (d.xadj[u + 1] - d.xadj[u]) <= d.hub_deg && reb_eligible<STRONG>(d, r, u);
-        if (active) conn_accumulate(d, gc, u, lane, G);
-        __syncthreads();
-        reb_eval<G, STRONG>(d, r, gc, u, lane, red, red2, active);
-        __syncthreads();
-        if (active) conn_reset(gc, *gc.tcount, lane, G);
-        __syncthreads();
-        if (active && lane == 0) *gc.tcount = 0;
-    }
-}
-
-template<bool STRONG>
-__global__ void reb_score_hub_kernel(JetDev d, RebDev r, const u32 *hubs) {
-    extern __shared__ __align__(16) char smem[];
-    __shared__ Best red[JB / WAVE];
-    __shared__ s64 red2[JB / WAVE];
-    const u32 u = hubs[blockIdx.x];
-    if (!reb_eligible<STRONG>(d, r, u)) return;
-    GroupConn gc = group_conn(smem, d.k, 0);
-    for (u32 i = threadIdx.x; i < d.k; i += JB) gc.cnt[i] = 0;
-    if (threadIdx.x == 0) *gc.tcount = 0;
-    __syncthreads();
-    conn_accumulate(d, gc, u, threadIdx.x, JB);
-    __syncthreads();
-    reb_eval<JB, STRONG>(d, r, gc, u, threadIdx.x, red, red2, true);
-}
-
-struct FlagSrc {
-    const u32 *flag;
-    __device__ u32 operator()(u64 u) const { return flag[u]; }
-};
-struct FlagDst {
-    const u32 *flag;
-    const u32 *gid;
-    u32 *keys, *vals;
-    __device__ void operator()(u64 u, u32 pos) const {
-        if (flag[u]) {
-            keys[pos] = gid[u];
-            vals[pos] = (u32) u;
-        }
-    }
-};
 struct WeightSrc {
     const u32 *sv;
     const s32 *vw;
@@ -669,107 +1325,74 @@ struct StoreS {
     __device__ void operator()(u64 i, s64 v) const { S[i] = v; }
 };
 
-// weak filter (jet:718-744) / strong filter (jet:472-500): score = S[i] - P[block begin]
-template<bool STRONG>
-__global__ void reb_select_kernel(JetDev d, RebDev r, const u32 *sv, const s64 *S, const s64 *P, u32 len) {
-    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < len; i0 += (u64) gridDim.x * blockDim.x) {
+// prefix at the first entry of every minibucket present
+__global__ void fb_head_kernel(const u64 *sk, const s64 *S, u32 n, s64 *H) {
+    for (u64 i = (u64) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64) gridDim.x * blockDim.x) {
+        const u32 g = (u32) (sk[i] >> 32);
+        if (i == 0 || (u32) (sk[i - 1] >> 32) != g) H[g] = S[i];
+    }
+}
+
+__global__ void fb_select_kernel(JetDev d, const u64 *sk, const s64 *S, const s64 *H, u32 n, s64 *roff,
+                                 int strong) {
+    JetCtl *ctl = d.ctl;
+    const u32 stamp = ctl->stamp;
+    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < n; i0 += (u64) gridDim.x * blockDim.x) {
         const u64 i = i0 + threadIdx.x;
         bool sel = false;
         u32 u = 0;
-        if (i < len) {
-            u = sv[i];
+        if (i < n) {
+            u = (u32) sk[i];
+            const u32 g = (u32) (sk[i] >> 32);
             const u32 ub = d.work[u];
-            const s64 score = S[i] - P[(u64) ub * r.width];
+            const s64 score = bucket_prefix(d, g, ub) + S[i] - H[g];
             const s64 limit = d.bw[ub] - d.lmax;
             if (score < limit) {
                 sel = true;
-                if (STRONG) {
-                    if (score + vwt(d, u) >= limit) r.evict_adjust[ub] = score + vwt(d, u);
-                    r.gid[u] = (u32) i;  // position in the sorted order, for adjust_scores
-                } else {
-                    d.mark[u] = d.stamp;
-                    d.oldp[u] = ub;
+                d.mark[u] = stamp;
+                d.oldp[u] = ub;
+                if (strong) {
+                    roff[u] = score;
+                    if (score + vwt(d, u) >= limit) d.evict_adjust[ub] = score + vwt(d, u);
                 }
             }
         }
-        append(sel, u, d.moves, &d.st[ST_MOVES]);
+        append(sel, u, d.moves_x, &ctl->st[ST_MOVES]);
     }
 }
 
-// cookie cutter (jet:508-552), one thread (k is small)
-__global__ void cookie_kernel(JetDev d, RebDev r, const s64 *P) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    s64 prefix = 0;
-    for (u32 p = 0; p < d.k; ++p) {
-        s64 add = r.evict_adjust[p];
-        const u64 begin = (u64) MAX_BUCKETS * p * r.sections;
-        if (add == 0) add = P[begin + (u64) MAX_BUCKETS * r.sections] - P[begin];
-        r.evict_adjust[p] = P[begin] - prefix;
-        prefix += add;
-    }
-    *r.max_vwgt = prefix;  // overflow counter start
-    s64 run = 0;
-    r.evict_start[0] = 0;
-    for (u32 p = 0; p < d.k; ++p) {
-        if (r.max_b_w > d.bw[p]) run += r.max_b_w - d.bw[p];
-        r.evict_start[p + 1] = run;
-    }
+__global__ void fb_cookie_kernel(JetDev d) {
+    if (threadIdx.x < WAVE && blockIdx.x == 0) cookie_cutter(d, max_block_weight(d, true));
 }
 
-// adjust_scores (jet:556-586) first pass: vertices whose position falls well
-// inside a chunk get it; the others are flagged for the serial pass
-__global__ void adjust_kernel(JetDev d, RebDev r, const s64 *S, u32 *over_flag) {
-    const u32 len = (u32) d.st[ST_MOVES];
+__global__ void fb_adjust_kernel(JetDev d, const s64 *roff, u32 *over_flag) {
+    const u32 len = (u32) d.ctl->st[ST_MOVES];
     for (u64 i = (u64) blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (u64) gridDim.x * blockDim.x) {
-        const u32 u = d.moves[i];
+        const u32 u = d.moves_x[i];
         const u32 ub = d.work[u];
-        const s64 pos = S[r.gid[u]] - r.evict_adjust[ub];
-        const s64 wu = vwt(d, u);
-        s64 id = 0;
-        while (id <= (s64) d.k && r.evict_start[id] <= pos) id++;
-        id--;
-        d.mark[u] = d.stamp;
-        d.oldp[u] = ub;
-        if (id < (s64) d.k && wu / 2 <= r.evict_start[id + 1] - pos) {
-            d.dest[u] = (u32) id;
-        } else if (id < (s64) d.k) {
-            over_flag[u] = 1;
-        } else {
-            d.dest[u] = ub;
-        }
+        const s64 pos = roff[u] + d.evict_adjust[ub];
+        const u32 dst = chunk_of(d.evict_start, d.k, pos, vwt(d, u), ub);
+        if (dst == NO_MOVE) over_flag[u] = 1;
+        else d.dest[u] = dst;
     }
 }
 
-// serial overflow resolution in vertex id order
-__global__ void overflow_kernel(JetDev d, RebDev r, const s64 *S, const u32 *olist, const u32 *olen,
-                                u32 *over_flag) {
+__global__ void fb_overflow_kernel(JetDev d, const s64 *roff, const u32 *olist, const u32 *olen, u32 *over_flag) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const u32 len = *olen;
-    s64 counter = *r.max_vwgt;
+    s64 counter = *d.counter;
     for (u32 q = 0; q < len; ++q) {
         const u32 u = olist[q];
         over_flag[u] = 0;
         const u32 ub = d.work[u];
-        s64 pos = S[r.gid[u]] - r.evict_adjust[ub];
-        const s64 wu = vwt(d, u);
-        s64 id = 0;
-        bool done = false;
-        while (id < (s64) d.k) {
-            while (id <= (s64) d.k && r.evict_start[id] <= pos) id++;
-            id--;
-            if (id < (s64) d.k && wu / 2 <= r.evict_start[id + 1] - pos) {
-                d.dest[u] = (u32) id;
-                done = true;
-                break;
-            }
-            if (id < (s64) d.k) {
-                pos = counter;
-                counter += wu;
-            }
-        }
-        if (!done) d.dest[u] = ub;
+        const s64 pos = roff[u] + d.evict_adjust[ub];
+        d.dest[u] = overflow_dest(d.evict_start, d.k, pos, vwt(d, u), ub, counter);
     }
-    *r.max_vwgt = counter;
+    *d.counter = counter;
+}
+
+__global__ void unstall_kernel(JetCtl *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->stalled = 0;
 }
 
 struct OverSrc {
@@ -790,177 +1413,84 @@ struct JetHost {
     const DGraph &g;
     const GraphMeta &meta;
     JetDev d{};
-    RebDev r{};
-    unsigned long long *report = nullptr;  // device copy of the counters
-    u32 G = 16;
-    u32 groups_per_block = 16;
-    size_t conn_smem = 0;
-    bool trace = false;
-    // rebalancing scratch
-    u32 *skeys = nullptr, *svals = nullptr, *skeys2 = nullptr, *svals2 = nullptr;
-    s64 *S = nullptr, *P = nullptr;
+    u32 G = 16, Gs = 16, sweep_threads = JB;
+    bool big_resolve = false;
+    size_t sweep_smem = 0, hub_smem = 0;
+    s64 *H = nullptr, *roff = nullptr;
     u32 *over_flag = nullptr, *olist = nullptr, *olen = nullptr;
 
     JetHost(Ctx &cc, const DGraph &gg, const GraphMeta &mm) : c(cc), g(gg), meta(mm) {}
 
-    void begin_iteration() {
-        d.stamp++;
-        fill<unsigned long long>(c, d.st, ST_COUNT, 0ull);
-    }
-
     template<int GG>
-    void launch_lp_miss(double ratio) {
-        launch(c, "jet_lp_best_block", 0.0, dim3(grid_for(c, g.n, groups_per_block, 8)),
-               dim3(groups_per_block * GG), conn_smem, lp_miss_kernel<GG>, d, ratio);
+    void launch_sweep() {
+        launch(c, "jet_sweep", 0.0, dim3(d.nreg), dim3(sweep_threads), sweep_smem, sweep_kernel<GG>, d);
     }
     template<int GG>
-    void launch_afterburner() {
-        launch(c, "jet_afterburner", 0.0, dim3(grid_for(c, g.n, JB / GG, 8)), dim3(JB), 0, afterburner_kernel<GG>, d,
-               (int) trace);
+    void launch_post() {
+        launch(c, "jet_post", 0.0, dim3(d.nreg), dim3(JB), 0, post_kernel<GG>, d);
     }
     template<int GG>
     void launch_apply() {
-        launch(c, "jet_apply", 0.0, dim3(grid_for(c, g.n, JB / GG, 8)), dim3(JB), 0, apply_kernel<GG>, d,
-               (int) trace);
-    }
-    template<int GG>
-    void launch_reb_score(bool strong) {
-        if (strong)
-            launch(c, "jet_rebalance_score", 0.0, dim3(grid_for(c, g.n, groups_per_block, 8)),
-                   dim3(groups_per_block * GG), conn_smem, reb_score_kernel<GG, true>, d, r);
-        else
-            launch(c, "jet_rebalance_score", 0.0, dim3(grid_for(c, g.n, groups_per_block, 8)),
-                   dim3(groups_per_block * GG), conn_smem, reb_score_kernel<GG, false>, d, r);
+        launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_kernel<GG>, d);
     }
 
-    void apply_and_finish() {
+    void apply() {
+        if (meta.n_hubs)
+            launch(c, "jet_apply", 0.0, dim3(meta.n_hubs), dim3(JB), 0, apply_hub_kernel, d, (const u32 *) meta.hubs);
         HEIPA_DISPATCH_G(G, launch_apply);
-        if (meta.n_hubs)
-            launch(c, "jet_apply", 0.0, dim3(meta.n_hubs), dim3(JB), 0, apply_hub_kernel, d, (const u32 *) meta.hubs,
-                   (int) trace);
-        launch(c, "jet_finish", 0.0, dim3(1), dim3(256), 0, finish_kernel, d, report);
+        launch(c, "jet_control", 0.0, dim3(1), dim3(JB), 0, control_kernel, d);
     }
 
-    void lp(double ratio) {
-        begin_iteration();
-        // reads dest_cache, lock, gain_cache; writes dest, gain1 (all n-sized)
-        const double cached_bytes = 25.0 * g.n;
-        launch(c, "jet_lp_cached", cached_bytes, dim3(grid_for(c, g.n, 256, 8)), dim3(256), 0, lp_cached_kernel, d);
-        HEIPA_DISPATCH_G(G, launch_lp_miss, ratio);
+    void iteration() {
         if (meta.n_hubs)
-            launch(c, "jet_lp_best_block", 0.0, dim3(meta.n_hubs), dim3(JB), conn_group_bytes(g_k()), lp_hub_kernel,
-                   d, (const u32 *) meta.hubs, ratio);
-        HEIPA_DISPATCH_G(G, launch_afterburner);
-        if (meta.n_hubs)
-            launch(c, "jet_afterburner", 0.0, dim3(meta.n_hubs), dim3(JB), 0, afterburner_hub_kernel, d,
-                   (const u32 *) meta.hubs, (int) trace);
-        apply_and_finish();
-    }
-
-    u32 g_k() const { return d.k; }
-
-    void sections(u32 k) {
-        u32 s = MAX_SECTIONS;
-        u32 section_size = (g.n + s * k) / (s * k);
-        if (section_size < 4096) {
-            section_size = 4096;
-            s = (g.n + section_size * k) / (section_size * k);
-        }
-        r.sections = s;
-        r.t = MAX_BUCKETS * k * s;
-        r.width = MAX_BUCKETS * s;
-    }
-
-    // shared part of weak/strong: score, bucket prefix, stable sort, S
-    u32 score_and_sort(bool strong, BackScope &bs) {
-        r.btot = bs.get<s64>((u64) r.t + 1);
-        P = bs.get<s64>((u64) r.t + 2);
-        fill<s64>(c, r.btot, (u64) r.t + 1, 0);
-        fill<u32>(c, r.flag, g.n, 0u);
-        HEIPA_DISPATCH_G(G, launch_reb_score, strong);
+            launch(c, "jet_sweep", 0.0, dim3(meta.n_hubs), dim3(JB), hub_smem, hub_sweep_kernel, d,
+                   (const u32 *) meta.hubs, 0);
+        HEIPA_DISPATCH_G(Gs, launch_sweep);
         if (meta.n_hubs) {
-            if (strong)
-                launch(c, "jet_rebalance_score", 0.0, dim3(meta.n_hubs), dim3(JB), conn_group_bytes(d.k),
-                       reb_score_hub_kernel<true>, d, r, (const u32 *) meta.hubs);
-            else
-                launch(c, "jet_rebalance_score", 0.0, dim3(meta.n_hubs), dim3(JB), conn_group_bytes(d.k),
-                       reb_score_hub_kernel<false>, d, r, (const u32 *) meta.hubs);
+            launch(c, "jet_sweep", 0.0, dim3(meta.n_hubs), dim3(JB), hub_smem, hub_sweep_kernel, d,
+                   (const u32 *) meta.hubs, 1);
+            launch(c, "jet_post", 0.0, dim3(meta.n_hubs), dim3(JB), 0, hub_post_kernel, d, (const u32 *) meta.hubs);
         }
-        exclusive_sum<s64>(c, "jet_rebalance_scan", r.btot, P, (u64) r.t + 1);
-        u32 *ncand = olen + 1;
-        scan<u32>(c, "jet_rebalance_compact", FlagSrc{r.flag}, FlagDst{r.flag, r.gid, skeys, svals}, g.n, ncand);
-        u32 len;
-        readback(c, &len, ncand, 1);
-        if (len == 0) return 0;
-        const bool flip = radix_sort_pairs<u32>(c, skeys, svals, skeys2, svals2, len, bits_for((u64) r.t + 1));
-        const u32 *sv = flip ? svals2 : svals;
-        scan<s64>(c, "jet_rebalance_scan", WeightSrc{sv, g.vw}, StoreS{S}, len, (s64 *) nullptr);
+        launch(c, "jet_buckets", 0.0, dim3(d.k), dim3(JB), 0, bucket_kernel, d);
+        HEIPA_DISPATCH_G(G, launch_post);
+        if (big_resolve)
+            launch(c, "jet_resolve", 0.0, dim3(1), dim3(RB), 0, resolve_kernel<RESOLVE_CAP>, d);
+        else
+            launch(c, "jet_resolve", 0.0, dim3(1), dim3(RB), 0, resolve_kernel<RESOLVE_CAP_SMALL>, d);
+        apply();
+    }
+
+    // the stalled iteration's resolve on the global path, then its apply
+    void fallback(const JetCtl &s) {
+        BackScope bs(c.arena);
+        const bool strong = s.mode == 2;
+        const u32 L = (u32) s.st[ST_RLIST];
+        u64 *k2 = bs.get<u64>(L);
+        u32 *v1 = bs.get<u32>(L), *v2 = bs.get<u32>(L);
+        s64 *S = bs.get<s64>(L);
+        const char *name = "jet_rebalance_global";
+        const dim3 grid(grid_for(c, L, 256));
+        launch(c, name, 0.0, grid, dim3(256), 0, fb_vals_kernel, (const u64 *) d.rlist, L, v1);
+        const bool flip = radix_sort_pairs<u64>(c, d.rlist, v1, k2, v2, L, 32 + bits_for((u64) d.t + 1));
+        const u64 *sk = flip ? k2 : d.rlist;
+        const u32 *sv = flip ? v2 : v1;
+        scan<s64>(c, name, WeightSrc{sv, g.vw}, StoreS{S}, L, (s64 *) nullptr);
+        launch(c, name, 0.0, grid, dim3(256), 0, fb_head_kernel, sk, (const s64 *) S, L, H);
+        if (strong) fill<s64>(c, d.evict_adjust, d.k, 0);
+        launch(c, name, 0.0, grid, dim3(256), 0, fb_select_kernel, d, sk, (const s64 *) S, (const s64 *) H, L, roff,
+               (int) strong);
         if (strong) {
-            fill<s64>(c, r.evict_adjust, d.k, 0);
-            launch(c, "jet_rebalance_select", 0.0, dim3(grid_for(c, len, 256)), dim3(256), 0,
-                   reb_select_kernel<true>, d, r, sv, (const s64 *) S, (const s64 *) P, len);
-        } else {
-            launch(c, "jet_rebalance_select", 0.0, dim3(grid_for(c, len, 256)), dim3(256), 0,
-                   reb_select_kernel<false>, d, r, sv, (const s64 *) S, (const s64 *) P, len);
-        }
-        return len;
-    }
-
-    u32 weak() {
-        begin_iteration();
-        BackScope bs(c.arena);
-        r.opt = (g.gw + (s64) (d.k - 1)) / (s64) d.k;
-        s64 mbw = (s64) ((double) d.lmax * 0.99);
-        if (mbw < d.lmax - 100) mbw = d.lmax - 100;
-        r.max_b_w = mbw;
-        sections(d.k);
-        launch(c, "jet_rebalance_prepare", 0.0, dim3(1), dim3(1024), 0, weak_prepare_kernel, d, r);
-        const u32 n_cand = score_and_sort(false, bs);
-        apply_and_finish();
-        return n_cand;
-    }
-
-    u32 strong() {
-        begin_iteration();
-        BackScope bs(c.arena);
-        r.opt = (g.gw + (s64) (d.k - 1)) / (s64) d.k;
-        r.max_b_w = std::max(r.opt + 1, (s64) ((double) d.lmax * 0.99));
-        sections(d.k);
-        launch(c, "jet_rebalance_prepare", 0.0, dim3(1), dim3(64), 0, strong_prepare_kernel, d, r);
-        const u32 n_cand = score_and_sort(true, bs);
-        if (n_cand) {
-            launch(c, "jet_rebalance_cookie", 0.0, dim3(1), dim3(64), 0, cookie_kernel, d, r, (const s64 *) P);
-            launch(c, "jet_rebalance_adjust", 0.0, dim3(grid_for(c, g.n, 256)), dim3(256), 0, adjust_kernel, d, r,
-                   (const s64 *) S, over_flag);
-            scan<u32>(c, "jet_rebalance_overflow", OverSrc{over_flag}, OverDst{over_flag, olist}, g.n, olen);
-            launch(c, "jet_rebalance_overflow", 0.0, dim3(1), dim3(64), 0, overflow_kernel, d, r, (const s64 *) S,
+            launch(c, name, 0.0, dim3(1), dim3(64), 0, fb_cookie_kernel, d);
+            launch(c, name, 0.0, dim3(grid_for(c, L, 256)), dim3(256), 0, fb_adjust_kernel, d, (const s64 *) roff,
+                   over_flag);
+            scan<u32>(c, name, OverSrc{over_flag}, OverDst{over_flag, olist}, g.n, olen);
+            launch(c, name, 0.0, dim3(1), dim3(64), 0, fb_overflow_kernel, d, (const s64 *) roff,
                    (const u32 *) olist, (const u32 *) olen, over_flag);
         }
-        apply_and_finish();
-        return n_cand;
+        launch(c, name, 0.0, dim3(1), dim3(64), 0, unstall_kernel, d.ctl);
+        apply();
     }
 };
-
-// Algorithmic bytes of the data-dependent Jet kernels, from the counters of
-// the iteration (w = 1 when edge weights are stored):
-//   best block over misses M: 4(|M|+1)... per miss 8 (xadj) + 4 (own block)
-//     + 25 (lock, gain/dest caches, dest, gain1), per edge 4(1+w) + 4 (block of
-//     the neighbour, counted at most once per vertex: capped at 4n);
-//   afterburner over candidates C: per candidate 8 + 8 (own gain, blocks),
-//     per edge 4(1+w), neighbour gain/dest/block 16 per distinct vertex (cap 16n);
-//   apply over moves M: per move 16, per edge 4(1+w), neighbour mark/block and
-//     cache invalidation 16 per distinct vertex (cap 16n).
-void account_bytes(Ctx &c, const DGraph &g, u32 mode, const unsigned long long *rep) {
-    const double w = g.ew ? 2.0 : 1.0, n = g.n;
-    if (mode == 0) {
-        const double nm = (double) rep[ST_MISS], em = (double) rep[ST_MISS_DEG];
-        prof_bytes(c, "jet_lp_best_block", 37.0 * nm + 4.0 * w * em + std::min(4.0 * em, 4.0 * n));
-        const double nc = (double) rep[ST_CAND], ec = (double) rep[ST_CAND_DEG];
-        prof_bytes(c, "jet_afterburner", 16.0 * nc + 4.0 * w * ec + std::min(16.0 * ec, 16.0 * n));
-    }
-    const double nmv = (double) rep[ST_MOVES], emv = (double) rep[ST_MOVE_DEG];
-    prof_bytes(c, "jet_apply", 16.0 * nmv + 4.0 * w * emv + std::min(16.0 * emv, 16.0 * n));
-}
 
 } // namespace
 
@@ -977,11 +1507,13 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.n = n;
     d.k = k;
     d.lmax = lmax;
+    d.opt = (g.gw + (s64) (k - 1)) / (s64) k;
     d.xadj = g.xadj;
     d.adj = g.adj;
     d.ew = g.ew;
     d.vw = g.vw;
     d.hub_deg = meta.hub_deg;
+    d.part = part;
     d.work = bs.get<u32>(n);
     d.dest = bs.get<u32>(n);
     d.dcache = bs.get<u32>(n);
@@ -990,119 +1522,150 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.gcache = bs.get<s64>(n);
     d.gain1 = bs.get<s64>(n);
     d.lock = bs.get<u8>(n);
-    d.miss = bs.get<u32>(n);
-    d.cand = bs.get<u32>(n);
-    d.moves = bs.get<u32>(n);
     d.bw = bs.get<s64>(k);
-    d.st = bs.get<unsigned long long>(ST_COUNT);
-    d.stamp = 0;
-    h.report = bs.get<unsigned long long>(ST_COUNT);
-    h.r.under = bs.get<u32>(k);
-    h.r.n_under = bs.get<u32>(1);
-    h.r.gid = bs.get<u32>(n);
-    h.r.flag = bs.get<u32>(n);
-    h.r.max_vwgt = bs.get<s64>(1);
-    h.r.evict_adjust = bs.get<s64>(k);
-    h.r.evict_start = bs.get<s64>(k + 1);
-    h.skeys = bs.get<u32>(n);
-    h.svals = bs.get<u32>(n);
-    h.skeys2 = bs.get<u32>(n);
-    h.svals2 = bs.get<u32>(n);
-    h.S = bs.get<s64>(n);
+    // minibuckets (jet:364-371, :605-612)
+    {
+        u32 s = MAX_SECTIONS;
+        u32 section_size = (n + s * k) / (s * k);
+        if (section_size < 4096) {
+            section_size = 4096;
+            s = (n + section_size * k) / (section_size * k);
+        }
+        d.sections = s;
+        d.t = MAX_BUCKETS * k * s;
+        d.width = MAX_BUCKETS * s;
+    }
+    d.gid = bs.get<u32>(n);
+    d.btot = bs.get<s64>((u64) d.t + 1);
+    d.Q = bs.get<s64>(d.t);
+    d.R = bs.get<s64>(d.t);
+    d.Tseg = bs.get<s64>(k);
+    d.rlist = bs.get<u64>(n);
+    d.evict_adjust = bs.get<s64>(k);
+    d.evict_start = bs.get<s64>(k + 1);
+    d.counter = bs.get<s64>(1);
+    h.H = bs.get<s64>((u64) d.t + 1);
+    h.roff = bs.get<s64>(n);
     h.over_flag = bs.get<u32>(n);
     h.olist = bs.get<u32>(n);
     h.olen = bs.get<u32>(4);
-    h.trace = trace != nullptr;
-    // group geometry: G lanes per vertex, LDS per group for k counters
-    h.G = meta.G;
-    h.conn_smem = 0;
+    // control block, followed by the trace ring when tracing
+    const bool tracing = trace != nullptr;
+    const size_t ctl_bytes = (sizeof(JetCtl) + 255) & ~(size_t) 255;
+    const size_t snap_bytes = ctl_bytes + (tracing ? TRACE_RING * sizeof(heipa_jet_trace) : 0);
+    const size_t slot_bytes = (snap_bytes + 255) & ~(size_t) 255;
+    require(c.pinned_bytes >= 65536 + 2 * slot_bytes, "pinned staging buffer too small for the Jet control block");
+    char *ctl_mem = bs.get<char>(snap_bytes);
+    d.ctl = (JetCtl *) ctl_mem;
+    d.trace = tracing ? (heipa_jet_trace *) (ctl_mem + ctl_bytes) : nullptr;
+    d.ratios = bs.get<double>(MAX_RATIOS);
+    RatioTable rt{};
     {
-        const size_t per_group = conn_group_bytes(k);
-        require(per_group <= 64 * 1024, "k too large for the LDS block-connectivity counters");
-        u32 gpb = JB / h.G;
-        while (gpb > 1 && per_group * gpb > 48 * 1024) gpb >>= 1;
-        h.groups_per_block = gpb;
-        h.conn_smem = per_group * gpb;
+        const double ur[MAX_RATIOS] = {0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45,
+                                       0.40, 0.35, 0.30, 0.25, 0.20, 0.15, 0.10, 0.05};
+        if (ultra) {
+            for (u32 i = 0; i < MAX_RATIOS; ++i) rt.r[i] = ur[i];
+            rt.n = MAX_RATIOS;
+        } else {
+            rt.r[0] = level == 0 ? 0.25 : 0.75;
+            rt.n = 1;
+        }
+        d.n_ratios = rt.n;
+    }
+    h.big_resolve = c.max_lds >= 160 * 1024;
+    d.rcap = std::min<u32>(c.resolve_cap, h.big_resolve ? RESOLVE_CAP : RESOLVE_CAP_SMALL);
+    // lane-group geometry: G lanes per vertex; the sweep keeps one LDS
+    // k-counter array per group, so wide k shrinks the groups per workgroup and,
+    // if one wave's groups do not fit, widens the groups
+    h.G = meta.G;
+    {
+        const size_t cgb = conn_group_bytes(k);
+        require(cgb <= SWEEP_LDS, "k too large for the LDS block-connectivity counters");
+        u32 Gs = meta.G;
+        while (Gs < WAVE && (size_t) (WAVE / Gs) * cgb > SWEEP_LDS) Gs <<= 1;
+        u32 gpb = JB / Gs;
+        while (gpb > WAVE / Gs && gpb * cgb > SWEEP_LDS) gpb >>= 1;
+        h.Gs = Gs;
+        h.sweep_threads = gpb * Gs;
+        h.sweep_smem = cgb * gpb + 4ull * WAVE * (h.sweep_threads / WAVE) + 16 + 4ull * k;
+        h.hub_smem = cgb + 16 + 4ull * k;
+        require(h.sweep_smem <= 65536 && h.hub_smem <= 65536, "k too large for the LDS block-connectivity counters");
+    }
+    // list regions: sweep workgroup r visits at most `waves * ceil(tiles / (nreg * waves))`
+    // tiles of 64 vertices; its post workgroup may add its share of the extra
+    // candidate list (hubs) to its move region
+    {
+        const u32 waves = h.sweep_threads / WAVE;
+        const u64 tiles = div_up(n, WAVE);
+        d.nreg = grid_for(c, tiles, waves, 8);
+        const u64 per = (tiles + (u64) d.nreg * waves - 1) / ((u64) d.nreg * waves);
+        d.cap = (u32) (per * waves * WAVE);
+        d.cap_mv = d.cap + div_up(meta.n_hubs, d.nreg);
+        d.cand = bs.get<u32>((u64) d.nreg * d.cap);
+        d.moves = bs.get<u32>((u64) d.nreg * d.cap_mv);
+        d.cand_n = bs.get<u32>(d.nreg);
+        d.moves_n = bs.get<u32>(d.nreg);
+        d.pstat = bs.get<unsigned long long>((u64) d.nreg * PT_COUNT);
+        d.cand_x = bs.get<u32>(meta.n_hubs + 1);
+        d.moves_x = bs.get<u32>(n);
     }
     copy_d2d(c, d.work, part, 4ull * n);
     fill<u32>(c, d.dcache, n, NULL_PART);
     fill<u8>(c, d.lock, n, (u8) 0);
     fill<u32>(c, d.mark, n, 0u);
     fill<u32>(c, h.over_flag, n, 0u);
+    fill<s64>(c, d.btot, (u64) d.t + 1, 0);
     block_weights_dev(c, g, d.work, k, d.bw);
+    launch(c, "jet_init", 0.0, dim3(1), dim3(64), 0, ctl_init_kernel, d, rt, cut, maxw);
 
-    std::vector<double> ratios;
-    if (ultra) ratios = {0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45, 0.40, 0.35, 0.30, 0.25, 0.20, 0.15, 0.10, 0.05};
-    else ratios = {level == 0 ? 0.25 : 0.75};
-    s64 cur_cut = cut, cur_maxw = maxw, best_cut = cut, best_maxw = maxw;
-    unsigned long long rep[ST_COUNT];
-    u32 iters = 0;
-    for (u32 ri = 0; ri < ratios.size(); ++ri) {
-        u32 bal = 0, it = 0;
-        while (it < N_MAX_ITERATIONS) {
-            it += 1;
-            u32 mode;
-            u32 n_cand = 0;
-            if (cur_maxw <= lmax) {
-                mode = 0;
-                h.lp(ratios[ri]);
-            } else if (bal < N_MAX_WEAK_ITERATIONS) {
-                mode = 1;
-                n_cand = h.weak();
-            } else {
-                mode = 2;
-                n_cand = h.strong();
-            }
-            readback(c, rep, h.report, ST_COUNT);
-            const u32 n_moves = (u32) rep[ST_MOVES];
-            if (c.prof_on) account_bytes(c, g, mode, rep);
-            if (mode == 0) {
-                n_cand = (u32) rep[ST_CAND];
-                bal = 0;
-                if (n_moves == 0) break;
-            } else if (mode == 1) {
-                if (n_moves == 0) {
-                    bal = N_MAX_WEAK_ITERATIONS;
-                    continue;
-                }
-                bal++;
-            } else {
-                if (n_moves == 0) break;
-                bal++;
-            }
-            ++iters;
-            cur_cut += (s64) rep[ST_DELTA] / 2;
-            cur_maxw = (s64) rep[ST_MAXW];
-            if (trace) {
-                heipa_jet_trace t{};
-                t.ratio_idx = ri;
-                t.iteration = it;
-                t.mode = mode;
-                t.n_cand = n_cand;
-                t.n_moves = n_moves;
-                t.cut = cur_cut;
-                t.maxw = cur_maxw;
-                t.moves_hash = rep[ST_MHASH];
-                t.gain_hash = mode == 0 ? rep[ST_GHASH] : 0;
-                trace->push_back(t);
-            }
-            if (best_maxw > lmax && cur_maxw < best_maxw) {
-                copy_d2d(c, part, d.work, 4ull * n);
-                best_cut = cur_cut;
-                best_maxw = cur_maxw;
-                it = 0;
-            } else if (cur_cut < best_cut && (cur_maxw <= lmax || cur_maxw <= best_maxw)) {
-                if ((double) cur_cut < PHI * (double) best_cut) it = 0;
-                copy_d2d(c, part, d.work, 4ull * n);
-                best_cut = cur_cut;
-                best_maxw = cur_maxw;
-            }
+    // Batches of iterations; the host reads the control block (and the trace
+    // ring) of batch i while batch i + 1 is already queued.
+    char *slot[2] = {(char *) c.pinned + 65536, (char *) c.pinned + 65536 + slot_bytes};
+    hipEvent_t ev[2] = {c.get_event(), c.get_event()};
+    u32 B = 2, consumed = 0;
+    u64 enqueued = 0;
+    auto submit = [&](int sidx) {
+        for (u32 i = 0; i < B; ++i) h.iteration();
+        enqueued += B;
+        if (enqueued > (u64) 1 << 22) throw Error(HEIPA_ERR_DEVICE, "jet_refine: iteration guard exceeded");
+        copy_d2h(c, slot[sidx], ctl_mem, snap_bytes);
+        HIP_CHECK(hipEventRecord(ev[sidx], c.stream));
+    };
+    JetCtl s{};
+    int pend = 0;
+    submit(0);
+    for (;;) {
+        submit(pend ^ 1);
+        HIP_CHECK(hipEventSynchronize(ev[pend]));
+        std::memcpy(&s, slot[pend], sizeof(JetCtl));
+        if (tracing) {
+            const heipa_jet_trace *ring = (const heipa_jet_trace *) (slot[pend] + ctl_bytes);
+            for (u32 i = consumed; i < s.n_trace; ++i) trace->push_back(ring[i % TRACE_RING]);
+            consumed = s.n_trace;
         }
+        if (!s.active) break;
+        if (s.stalled) {
+            HIP_CHECK(hipEventSynchronize(ev[pend ^ 1]));
+            h.fallback(s);
+            submit(pend);
+            continue;
+        }
+        pend ^= 1;
+        B = std::min<u32>(B * 2, BATCH_MAX);
     }
     c.sync();
-    res.cut = best_cut;
-    res.maxw = best_maxw;
-    res.iterations = iters;
+    c.event_pool.push_back(ev[0]);
+    c.event_pool.push_back(ev[1]);
+    if (s.save) copy_d2d(c, part, d.work, 4ull * n);
+    prof_bytes(c, "jet_sweep", s.bytes[0]);
+    prof_bytes(c, "jet_buckets", s.bytes[1]);
+    prof_bytes(c, "jet_post", s.bytes[2]);
+    prof_bytes(c, "jet_resolve", s.bytes[3]);
+    prof_bytes(c, "jet_apply", s.bytes[4]);
+    c.sync();
+    res.cut = s.best_cut;
+    res.maxw = s.best_maxw;
+    res.iterations = s.iters;
     return res;
 }
 

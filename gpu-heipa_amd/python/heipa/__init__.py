@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libheipa.so")
 HEADER_PATH = os.path.abspath(os.path.join(PKG_ROOT, "..", "include", "heipa.h"))
 
 OK, ERR_OOM, ERR_INVALID, ERR_OVERFLOW, ERR_DEVICE, ERR_IO = 0, 1, 2, 3, 4, 5
+OPT_RESOLVE_CAP = 1
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -107,6 +108,7 @@ def lib():
                                         ctypes.POINTER(LevelTrace), c_u32]),
             "heipa_hm_partition": (c_int, [vp, GP, u32p, c_u32, c_dbl, c_u64, c_int, vp]),
             "heipa_adaptive_imbalance": (c_dbl, [c_dbl, ctypes.c_int64, c_u32, ctypes.c_int64, c_u32, c_u64]),
+            "heipa_ctx_set_option": (c_int, [vp, c_int, c_u64]),
             "heipa_prof_enable": (c_int, [vp, c_int]),
             "heipa_prof_reset": (c_int, [vp]),
             "heipa_prof_filter": (c_int, [vp, ctypes.c_char_p]),
@@ -436,6 +438,9 @@ class Context:
         adjwgt = dl(desc.adjwgt, m, np.int32) if desc.adjwgt else None
         vwgt = dl(desc.vwgt, n, np.int32) if desc.vwgt else None
         return HostGraph(xadj, adjncy, adjwgt, vwgt)
+
+    def set_option(self, option, value):
+        self._check("heipa_ctx_set_option", self.L.heipa_ctx_set_option(self.h, int(option), int(value)))
 
     # per-kernel timing
     def prof_enable(self, on=True):

@@ -147,6 +147,14 @@ int heipa_hm_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hie
 double heipa_adaptive_imbalance(double global_imbalance, int64_t global_g_weight, uint32_t global_k,
                                 int64_t local_g_weight, uint32_t local_k_rem, uint64_t depth);
 
+/* ---- context options.  No counterpart in the reference (its sizes are compile-time).
+ * HEIPA_OPT_RESOLVE_CAP: most rebalance resolve entries one workgroup sorts in LDS before an
+ * iteration takes the device-wide radix-sort path (default and maximum: what the device's LDS
+ * allows; 0 forces the device-wide path).  Results do not depend on it; tests use it to cover
+ * both paths. */
+#define HEIPA_OPT_RESOLVE_CAP 1
+int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
+
 /* ---- per-kernel timing (HIP events on the context stream) */
 int heipa_prof_enable(heipa_ctx *ctx, int on);
 /* time only kernels whose name starts with `prefix` (NULL or "" = all) */

@@ -12,9 +12,17 @@ from graphs import block_weights, random_partition, star_plus_grid
 pytestmark = pytest.mark.gpu
 
 
+def _vertex_weighted(g, hi, seed):
+    """Same structure with vertex weights uniform in [1, hi] (coarse levels of
+    the solver are vertex-weighted; the generators are not)."""
+    vw = np.random.default_rng(seed).integers(1, hi + 1, g.n).astype(np.int32)
+    return heipa.HostGraph(g.xadj, g.adjncy, g.adjwgt, vw)
+
+
 def _graphs():
     return {
         "grid64": heipa.gen_grid(64, 64),
+        "rgg12vw": _vertex_weighted(heipa.gen_rgg(1 << 12, 10.0, 6, 10), 20, 3),
         "grid_rect": heipa.gen_grid(37, 91),
         "rgg14w": heipa.gen_rgg(1 << 14, 12.0, 1, 10),
         "rgg12u": heipa.gen_rgg(1 << 12, 8.0, 5, 0),
@@ -109,6 +117,19 @@ def test_jet(ctx, graph, ultra, level, skew, k):
     gp = heipa.to_numpy_u32(pd)
     assert np.array_equal(gp, op), name
     assert orc.edge_cut(g, gp) == gcut
+
+
+@pytest.mark.parametrize("cap", [0, 16])
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 1, 0.7, 7)])
+def test_jet_global_resolve(ctx, graph, cap, ultra, level, skew, k):
+    """Rebalance iterations whose resolve list exceeds the LDS capacity take the
+    device-wide radix-sort path; a capacity of 0 sends every rebalance there, 16
+    mixes both paths within one refinement."""
+    ctx.set_option(heipa.OPT_RESOLVE_CAP, cap)
+    try:
+        test_jet(ctx, graph, ultra, level, skew, k)
+    finally:
+        ctx.set_option(heipa.OPT_RESOLVE_CAP, 1 << 32)
 
 
 @pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False), (3, False), (7, True)])
