@@ -58,6 +58,7 @@ constexpr u32 ES_CAP = 1025;             // eviction chunk bounds kept in LDS (k
 constexpr u32 TRACE_RING = 256;          // trace records kept on the device between host reads
 constexpr u32 MAX_RATIOS = 17;
 constexpr size_t SWEEP_LDS = 48 * 1024;  // block-connectivity counters per sweep workgroup
+constexpr u32 SWEEP_PIECE = 1024;        // vertices listed in LDS per sweep phase
 constexpr u32 BATCH_MAX = 8;             // iterations enqueued per host read of the control block
 
 // float thresholds: LO[k] = least float >= 1.5^-k, HI[k] = greatest float <= 1.5^k
@@ -136,6 +137,7 @@ struct JetDev {
     const s32 *ew, *vw;
     u32 *work, *dest, *dcache, *mark, *oldp, *part;
     s64 *gcache, *gain1;
+    s64 *icache;  // connection to the own block of a vertex whose neighbours all share its block; -1: unknown
     u8 *lock;
     u32 *cand, *moves;      // regions of cap / cap_mv entries, one per sweep workgroup
     u32 *cand_n, *moves_n;  // region lengths
@@ -336,6 +338,7 @@ __device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u3
     bool cand = false;
     if (active && lane == 0) {
         const s64 own = (s64) gc.cnt[own_b];
+        d.icache[u] = (T == 0 || (T == 1 && gc.touched[0] == own_b)) ? own : -1;
         u32 best_id = best.b;
         s64 gain = 0;
         if (best_id != NO_MOVE) {
@@ -422,6 +425,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
         best = best_reduce<G>(best, red);
         if (active && lane == 0) {
             const s64 own = (s64) gc.cnt[own_b];
+            d.icache[u] = (T == 0 || (T == 1 && gc.touched[0] == own_b)) ? own : -1;
             u32 dst;
             s64 gain;
             if (best.b != NO_MOVE && best.c > 0) {
@@ -455,6 +459,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
         count = group_sum<G>(count, red2);
         if (active && lane == 0) {
             const s64 own = (s64) gc.cnt[own_b];
+            d.icache[u] = (T == 0 || (T == 1 && gc.touched[0] == own_b)) ? own : -1;
             if (count == 0) count = 1;
             const s64 gain = sum / count - own;
             const s64 wu = vwt(d, u);
@@ -469,6 +474,30 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
     return listed;
 }
 
+// reb_eval of an eligible vertex whose neighbours all lie in its own block
+// (own connection `own`), without its adjacency: no other block is touched,
+// so weak sends it to underloaded[u % count] with gain -own and strong
+// scores it 0 / 1 - own.  Returns whether u is a candidate.
+__device__ __forceinline__ bool reb_interior(const JetDev &d, bool strong, const RebShared *rs, u32 u, s64 own) {
+    const u32 own_b = d.work[u];
+    const s64 wu = vwt(d, u);
+    if (!strong) {
+        const u32 nu = rs->n_under;
+        const u32 dst = nu ? rs->under[u % nu] : own_b;
+        d.dest[u] = dst;
+        if (dst == own_b) return false;
+        const u32 g = (MAX_BUCKETS * own_b + gain_bucket(-own, wu)) * d.sections + (u % d.sections);
+        d.gid[u] = g;
+        atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) wu);
+        return true;
+    }
+    const s64 lim = d.bw[own_b] - d.lmax;
+    const u32 g = (MAX_BUCKETS * own_b + gain_bucket(-own, wu < lim ? wu : lim)) * d.sections + (u % d.sections) + 1;
+    d.gid[u] = g;
+    atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) wu);
+    return true;
+}
+
 // score offset of minibucket g for a vertex of block ub: the weight of the
 // candidates in minibuckets [ub * width, g).  A strong candidate in the last
 // minibucket of its block lands at index (ub + 1) * width (the reference's +1
@@ -478,73 +507,78 @@ __device__ __forceinline__ s64 bucket_prefix(const JetDev &d, u32 g, u32 ub) {
 }
 
 // ---------------------------------------------------------------- sweep
-// Each wave walks tiles of 64 consecutive vertices.  LP: cached vertices take
-// their cached decision lane by lane; rebuilt vertices are listed in the
-// wave's LDS slot and evaluated by the wave's 64/G lane groups.  Rebalance:
-// eligible vertices are listed and scored the same way.  Hubs are left to
-// hub_sweep_kernel.  A pending best-partition copy is done on the way.
+// Workgroup r owns vertices [r * cap, (r + 1) * cap), taken in pieces of up to
+// SWEEP_PIECE.  Phase A streams the piece, one vertex per thread: LP takes
+// the cached decision of cached vertices, rebalance tests eligibility; the
+// vertices to rebuild or score go to an LDS list.  Phase B hands that list to
+// the workgroup's lane groups, G lanes per vertex.  Hubs are left to
+// hub_sweep_kernel.  A pending best-partition copy is done in phase A.
 template<int G>
 __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
     extern __shared__ __align__(16) char smem[];
-    __shared__ u32 s_cnt;
+    __shared__ u32 s_cnt, s_nlist;
     __shared__ u64 s_red[JB / WAVE];
     JetCtl *ctl = d.ctl;
     if (!live(ctl)) return;
     u32 *region = d.cand + (u64) blockIdx.x * d.cap;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
     const u32 mode = ctl->mode;
     const bool save = ctl->save != 0;
     const double ratio = ctl->ratio;
-    constexpr u32 GPW = WAVE / G;
-    const u32 groups = blockDim.x / G, waves = blockDim.x / WAVE;
-    const u32 wid = threadIdx.x / WAVE, l64 = lane_id();
-    const u32 lane = threadIdx.x % G, gsub = l64 / G;
+    const u32 groups = blockDim.x / G;
+    const u32 lane = threadIdx.x % G, gi = threadIdx.x / G;
     const size_t cgb = conn_group_bytes(d.k);
-    GroupConn gc = group_conn(smem, d.k, threadIdx.x / G);
-    u32 *wlist = (u32 *) (smem + cgb * groups) + wid * WAVE;
-    RebShared *rs = (RebShared *) (smem + cgb * groups + 4ull * WAVE * waves);
+    GroupConn gc = group_conn(smem, d.k, gi);
+    u32 *list = (u32 *) (smem + cgb * groups);
+    RebShared *rs = (RebShared *) (smem + cgb * groups + 4ull * SWEEP_PIECE);
     for (u32 i = lane; i < d.k; i += G) gc.cnt[i] = 0;
     if (lane == 0) *gc.tcount = 0;
+    if (threadIdx.x == 0) s_cnt = 0;
     const bool strong = mode == 2;
     const s64 mbw = mode ? max_block_weight(d, strong) : 0;
     if (mode) reb_prologue(d, strong, mbw, rs);
-    wave_sync();
+    __syncthreads();
     u64 nev = 0, degs = 0;
-    const u64 stride = (u64) gridDim.x * waves * WAVE;
-    for (u64 base = ((u64) blockIdx.x * waves + wid) * WAVE; base < d.n; base += stride) {
-        const u64 vv = base + l64;
-        const u32 v = (u32) vv;
-        bool heavy = false, cand = false;
-        if (vv < d.n) {
-            if (save) d.part[v] = d.work[v];
-            if (mode == 0) {
-                const u32 dc = d.dcache[v];
-                if (dc == NULL_PART) {
-                    heavy = degree(d, v) <= d.hub_deg;
-                } else {
-                    d.dest[v] = dc;
-                    if (dc != NO_MOVE && d.lock[v] == 0) {
-                        cand = true;
-                        d.gain1[v] = d.gcache[v];
+    const u64 vbegin = (u64) blockIdx.x * d.cap;
+    const u64 vend = vbegin + d.cap < d.n ? vbegin + d.cap : d.n;
+    for (u64 p0 = vbegin; p0 < vend; p0 += SWEEP_PIECE) {
+        const u64 p1 = p0 + SWEEP_PIECE < vend ? p0 + SWEEP_PIECE : vend;
+        if (threadIdx.x == 0) s_nlist = 0;
+        __syncthreads();
+        for (u64 v0 = p0; v0 < p1; v0 += blockDim.x) {
+            const u64 vv = v0 + threadIdx.x;
+            const u32 v = (u32) vv;
+            bool heavy = false, cand = false;
+            if (vv < p1) {
+                if (save) d.part[v] = d.work[v];
+                if (mode == 0) {
+                    const u32 dc = d.dcache[v];
+                    if (dc == NULL_PART) {
+                        heavy = degree(d, v) <= d.hub_deg;
                     } else {
-                        d.gain1[v] = GAIN_MIN;
-                        d.lock[v] = 0;
+                        d.dest[v] = dc;
+                        if (dc != NO_MOVE && d.lock[v] == 0) {
+                            cand = true;
+                            d.gain1[v] = d.gcache[v];
+                        } else {
+                            d.gain1[v] = GAIN_MIN;
+                            d.lock[v] = 0;
+                        }
                     }
+                } else if (degree(d, v) <= d.hub_deg && reb_eligible(d, strong, rs, v)) {
+                    const s64 ic = d.icache[v];
+                    if (ic >= 0) cand = reb_interior(d, strong, rs, v, ic);
+                    else heavy = true;
                 }
-            } else {
-                heavy = degree(d, v) <= d.hub_deg && reb_eligible(d, strong, rs, v);
             }
+            append_local(cand, v, region, &s_cnt);
+            append_local(heavy, v, list, &s_nlist);
         }
-        append_local(cand, v, region, &s_cnt);
-        const u64 hm = __ballot(heavy);
-        const u32 nh = (u32) __popcll(hm);
-        if (heavy) wlist[__popcll(hm & lanemask_lt())] = v;
-        wave_sync();
-        for (u32 r0 = 0; r0 < nh; r0 += GPW) {
-            const u32 idx = r0 + gsub;
-            const bool active = idx < nh;
-            const u32 u = active ? wlist[idx] : 0;
+        __syncthreads();
+        const u32 nl = s_nlist;
+        for (u32 r0 = 0; r0 < nl; r0 += groups) {
+            const u32 idx = r0 + gi;
+            const bool active = idx < nl;
+            const u32 u = active ? list[idx] : 0;
             if (active) {
                 if (lane == 0) {
                     nev++;
@@ -562,6 +596,7 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
             wave_sync();
             append_local(listed, u, region, &s_cnt);
         }
+        __syncthreads();
     }
     nev = block_total(nev, s_red);
     degs = block_total(degs, s_red);
@@ -668,14 +703,14 @@ __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 la
     return change;
 }
 
+// afterburner over `len` candidates of `region`, G lanes each; selected
+// vertices go to `mregion` (LDS counter); per-thread statistics in degs / gh
 template<int G>
-__device__ __forceinline__ void afterburner_body(const JetDev &d, u32 stamp, u32 *mregion, u32 *s_cnt, u64 *s_red) {
+__device__ __forceinline__ void afterburner_list(const JetDev &d, u32 stamp, const u32 *region, u32 len, u32 *mregion,
+                                                 u32 *s_cnt, u64 &degs, u64 &gh) {
     const bool do_hash = d.trace != nullptr;
     const u32 groups = blockDim.x / G;
     const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
-    const u32 *region = d.cand + (u64) blockIdx.x * d.cap;
-    const u32 len = d.cand_n[blockIdx.x];
-    u64 gh = 0, degs = 0;
     for (u32 i0 = 0; i0 < len; i0 += groups) {
         const u32 i = i0 + gi;
         const bool active = i < len;
@@ -697,6 +732,12 @@ __device__ __forceinline__ void afterburner_body(const JetDev &d, u32 stamp, u32
         }
         append_local(sel, u, mregion, s_cnt);
     }
+}
+
+template<int G>
+__device__ __forceinline__ void afterburner_body(const JetDev &d, u32 stamp, u32 *mregion, u32 *s_cnt, u64 *s_red) {
+    u64 gh = 0, degs = 0;
+    afterburner_list<G>(d, stamp, d.cand + (u64) blockIdx.x * d.cap, d.cand_n[blockIdx.x], mregion, s_cnt, degs, gh);
     degs = block_total(degs, s_red);
     gh = block_total(gh, s_red);
     if (threadIdx.x == 0) {
@@ -871,25 +912,50 @@ __device__ void cookie_cutter(const JetDev &d, s64 mbw) {
 // runs the cookie cutter and adjust_scores.  Lists longer than CAP, or more
 // than OVF_CAP overflowing vertices, stall the loop for the host's global path
 // (nothing has been appended to the move list at that point).
+struct ResolveLds {
+    u64 *skey;
+    s64 *soff;
+    u8 *ssel;
+    s64 *red;    // RB / WAVE
+    u32 *ocount;
+    u32 *olist;  // OVF_CAP
+    s64 *s_es;   // ES_CAP
+};
+
+// LDS bytes of ResolveLds for CAP entries (16-byte aligned parts)
+__host__ __device__ constexpr size_t resolve_lds_bytes(u32 cap) {
+    return 16ull * cap + ((cap + 15) & ~15u) + 8ull * (RB / WAVE) + 16 + 4ull * OVF_CAP + 8ull * ES_CAP;
+}
+__device__ inline ResolveLds resolve_lds(char *base, u32 cap) {
+    ResolveLds r;
+    r.skey = (u64 *) base;
+    r.soff = (s64 *) (base + 8ull * cap);
+    r.ssel = (u8 *) (base + 16ull * cap);
+    char *p = base + 16ull * cap + ((cap + 15) & ~15u);
+    r.red = (s64 *) p;
+    p += 8ull * (RB / WAVE);
+    r.ocount = (u32 *) p;
+    p += 16;
+    r.olist = (u32 *) p;
+    p += 4ull * OVF_CAP;
+    r.s_es = (s64 *) p;
+    return r;
+}
+
+// Resolves a list of L entries with RB threads; false: the list (or the
+// strong overflow) does not fit and nothing was appended.  Moves go to the
+// extra move list, counted in LDS (`local_cnt`) or in st[ST_MOVES].
 template<u32 CAP>
-__global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
-    __shared__ u64 skey[CAP];
-    __shared__ s64 soff[CAP];
-    __shared__ u8 ssel[CAP];
-    __shared__ s64 red[RB / WAVE];
-    __shared__ u32 ocount;
-    __shared__ u32 olist[OVF_CAP];
-    __shared__ s64 s_es[ES_CAP];
-    JetCtl *ctl = d.ctl;
-    if (!live(ctl) || ctl->mode == 0) return;
-    const bool strong = ctl->mode == 2;
-    const u32 stamp = ctl->stamp;
-    const u32 L = (u32) ctl->st[ST_RLIST];
-    if (L == 0) return;
-    if (L > CAP || L > d.rcap) {
-        if (threadIdx.x == 0) ctl->stalled = 1;
-        return;
-    }
+__device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, const ResolveLds &lds, u32 *local_cnt) {
+    u64 *skey = lds.skey;
+    s64 *soff = lds.soff;
+    u8 *ssel = lds.ssel;
+    s64 *red = lds.red;
+    u32 &ocount = *lds.ocount;
+    u32 *olist = lds.olist;
+    s64 *s_es = lds.s_es;
+    if (L == 0) return true;
+    if (L > CAP || L > d.rcap) return false;
     u32 P2 = 1;
     while (P2 < L) P2 <<= 1;
     if (threadIdx.x == 0) ocount = 0;
@@ -991,10 +1057,7 @@ __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
         }
         __syncthreads();
         const u32 oc = ocount;
-        if (oc > OVF_CAP) {
-            if (threadIdx.x == 0) ctl->stalled = 1;
-            return;
-        }
+        if (oc > OVF_CAP) return false;
         if (threadIdx.x == 0 && oc) {
             // serial overflow in vertex id order (insertion sort of the entries by id)
             for (u32 a = 1; a < oc; ++a) {
@@ -1022,8 +1085,20 @@ __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
     for (u32 i0 = 0; i0 < L; i0 += RB) {
         const u32 i = i0 + threadIdx.x;
         const bool take = i < L && ssel[i] != 0;
-        append(take, take ? (u32) skey[i] : 0u, d.moves_x, &ctl->st[ST_MOVES]);
+        if (local_cnt) append_local(take, take ? (u32) skey[i] : 0u, d.moves_x, local_cnt);
+        else append(take, take ? (u32) skey[i] : 0u, d.moves_x, &d.ctl->st[ST_MOVES]);
     }
+    return true;
+}
+
+template<u32 CAP>
+__global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
+    __shared__ __align__(16) char smem[resolve_lds_bytes(CAP)];
+    JetCtl *ctl = d.ctl;
+    if (!live(ctl) || ctl->mode == 0) return;
+    const ResolveLds lds = resolve_lds(smem, CAP);
+    const bool ok = resolve_body<CAP>(d, ctl->mode == 2, ctl->stamp, (u32) ctl->st[ST_RLIST], lds, nullptr);
+    if (!ok && threadIdx.x == 0) ctl->stalled = 1;
 }
 
 // ---------------------------------------------------------------- apply moves and control
@@ -1041,6 +1116,7 @@ __device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 
         const s64 x = ewt(d, j) * ((s64) (nu != nv) - (s64) (ou != ov));
         delta += mv ? x : 2 * x;
         d.dcache[v] = NULL_PART;
+        d.icache[v] = -1;
     }
     return delta;
 }
@@ -1049,6 +1125,7 @@ __device__ __forceinline__ void move_commit(const JetDev &d, u32 u) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
     const s64 w = vwt(d, u);
     d.dcache[u] = NULL_PART;
+    d.icache[u] = -1;
     d.work[u] = nu;
     atomicAdd((unsigned long long *) &d.bw[ou], (unsigned long long) (-w));
     atomicAdd((unsigned long long *) &d.bw[nu], (unsigned long long) w);
@@ -1248,18 +1325,16 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
 }
 
 // one workgroup: the largest block weight and the sums of the per-workgroup
-// partials, then control_step
+// partials (one barrier), then control_step
 __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
-    __shared__ s64 red[JB / WAVE];
-    __shared__ u64 s_red[JB / WAVE];
+    constexpr u32 NV = PT_COUNT + 3;  // partials, two list lengths, max block weight
+    __shared__ u64 s_part[JB / WAVE][NV];
     if (!live(d.ctl)) return;
+    u64 t[NV];
+#pragma unroll
+    for (u32 j = 0; j < NV; ++j) t[j] = 0;
     s64 mx = INT64_MIN;
     for (u32 b = threadIdx.x; b < d.k; b += JB) mx = d.bw[b] > mx ? d.bw[b] : mx;
-    mx = wave_max(mx);
-    if (lane_id() == 0) red[threadIdx.x / WAVE] = mx;
-    u64 t[PT_COUNT + 2];
-#pragma unroll
-    for (u32 j = 0; j < PT_COUNT + 2; ++j) t[j] = 0;
     for (u32 r = threadIdx.x; r < d.nreg; r += JB) {
 #pragma unroll
         for (u32 j = 0; j < PT_COUNT; ++j) t[j] += d.pstat[(u64) r * PT_COUNT + j];
@@ -1267,12 +1342,273 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
         t[PT_COUNT + 1] += d.moves_n[r];
     }
 #pragma unroll
-    for (u32 j = 0; j < PT_COUNT + 2; ++j) t[j] = block_total(t[j], s_red);
+    for (u32 j = 0; j < NV - 1; ++j) t[j] = wave_sum(t[j]);
+    mx = wave_max(mx);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (u32 j = 0; j < NV - 1; ++j) s_part[threadIdx.x / WAVE][j] = t[j];
+        s_part[threadIdx.x / WAVE][NV - 1] = (u64) mx;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         s64 m = INT64_MIN;
-        for (u32 w = 0; w < JB / WAVE; ++w) m = red[w] > m ? red[w] : m;
+#pragma unroll
+        for (u32 j = 0; j < NV - 1; ++j) t[j] = 0;
+        for (u32 w = 0; w < JB / WAVE; ++w) {
+#pragma unroll
+            for (u32 j = 0; j < NV - 1; ++j) t[j] += s_part[w][j];
+            m = (s64) s_part[w][NV - 1] > m ? (s64) s_part[w][NV - 1] : m;
+        }
         control_step(d, m, t);
+    }
+}
+
+// ---------------------------------------------------------------- single-workgroup form
+// Small levels: one workgroup of BM threads runs whole iterations (sweep,
+// minibucket prefixes, afterburner or classify, resolve, apply, control)
+// with barriers where the multi-launch form has launch boundaries, up to
+// max_iters iterations per launch.  Same device functions, same results.
+// Lists: candidates in d.cand, moves in the extra move list, resolve entries
+// in d.rlist, all counted in LDS.  A resolve list that does not fit stalls
+// the loop: the iteration's lengths and statistics are published in st[] and
+// region counts are zero, so the host's global path and the multi-launch
+// apply and control finish that iteration.
+constexpr int BM = 1024;
+constexpr u32 BM_PIECE = 2048;
+constexpr u32 BM_RCAP = 2048;
+constexpr size_t BM_CONN_LDS = 32 * 1024;
+constexpr u32 BM_MAX_N = 1024;
+
+__host__ __device__ inline size_t bm_sweep_lds_bytes(u32 k, u32 groups) {
+    return conn_group_bytes(k) * groups + 4ull * BM_PIECE + 16 + 4ull * k;
+}
+
+template<int G>
+__global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, u32 groups) {
+    extern __shared__ __align__(16) char smem[];
+    __shared__ u32 s_live, s_mode, s_stamp, s_save;
+    __shared__ double s_ratio;
+    __shared__ u32 s_ncand, s_nlist, s_nmoves, s_nrl;
+    __shared__ u64 s_acc[PT_COUNT];
+    __shared__ u64 s_red[BM / WAVE];
+    JetCtl *ctl = d.ctl;
+    const u32 tid = threadIdx.x, lane = tid % G, gi = tid / G;
+    const bool ingroup = gi < groups;
+    const size_t cgb = conn_group_bytes(d.k);
+    GroupConn gc = group_conn(smem, d.k, ingroup ? gi : 0);
+    u32 *list = (u32 *) (smem + cgb * groups);
+    RebShared *rs = (RebShared *) (smem + cgb * groups + 4ull * BM_PIECE);
+    const ResolveLds rl = resolve_lds(smem, BM_RCAP);
+    for (u32 r = tid; r < d.nreg; r += BM) {
+        d.cand_n[r] = 0;
+        d.moves_n[r] = 0;
+    }
+    for (u64 j = tid; j < (u64) d.nreg * PT_COUNT; j += BM) d.pstat[j] = 0;
+    for (u32 iter = 0; iter < max_iters; ++iter) {
+        if (tid == 0) {
+            s_live = live(ctl);
+            s_mode = ctl->mode;
+            s_stamp = ctl->stamp;
+            s_save = ctl->save;
+            s_ratio = ctl->ratio;
+            s_ncand = s_nmoves = s_nrl = 0;
+            for (u32 j = 0; j < PT_COUNT; ++j) s_acc[j] = 0;
+        }
+        __syncthreads();
+        if (!s_live) break;
+        const u32 mode = s_mode, stamp = s_stamp;
+        const bool save = s_save != 0, strong = mode == 2;
+        const double ratio = s_ratio;
+        const s64 mbw = mode ? max_block_weight(d, strong) : 0;
+        // ---- sweep
+        if (ingroup) {
+            for (u32 i = lane; i < d.k; i += G) gc.cnt[i] = 0;
+            if (lane == 0) *gc.tcount = 0;
+        }
+        if (mode) reb_prologue(d, strong, mbw, rs);
+        __syncthreads();
+        u64 nev = 0, degs = 0;
+        for (u64 p0 = 0; p0 < d.n; p0 += BM_PIECE) {
+            const u64 p1 = p0 + BM_PIECE < d.n ? p0 + BM_PIECE : d.n;
+            if (tid == 0) s_nlist = 0;
+            __syncthreads();
+            for (u64 v0 = p0; v0 < p1; v0 += BM) {
+                const u64 vv = v0 + tid;
+                const u32 v = (u32) vv;
+                bool heavy = false, cand = false;
+                if (vv < p1) {
+                    if (save) d.part[v] = d.work[v];
+                    if (mode == 0) {
+                        const u32 dc = d.dcache[v];
+                        if (dc == NULL_PART) {
+                            heavy = true;
+                        } else {
+                            d.dest[v] = dc;
+                            if (dc != NO_MOVE && d.lock[v] == 0) {
+                                cand = true;
+                                d.gain1[v] = d.gcache[v];
+                            } else {
+                                d.gain1[v] = GAIN_MIN;
+                                d.lock[v] = 0;
+                            }
+                        }
+                    } else if (reb_eligible(d, strong, rs, v)) {
+                        const s64 ic = d.icache[v];
+                        if (ic >= 0) cand = reb_interior(d, strong, rs, v, ic);
+                        else heavy = true;
+                    }
+                }
+                append_local(cand, v, d.cand, &s_ncand);
+                append_local(heavy, v, list, &s_nlist);
+            }
+            __syncthreads();
+            const u32 nl = s_nlist;
+            for (u32 r0 = 0; r0 < nl; r0 += groups) {
+                const u32 idx = r0 + gi;
+                const bool active = ingroup && idx < nl;
+                const u32 u = active ? list[idx] : 0;
+                if (active) {
+                    if (lane == 0) {
+                        nev++;
+                        degs += degree(d, u);
+                    }
+                    conn_accumulate(d, gc, u, lane, G);
+                }
+                wave_sync();
+                const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, active)
+                                              : reb_eval<G>(d, strong, mbw, rs, gc, u, lane, nullptr, nullptr, active);
+                wave_sync();
+                if (active) conn_reset(gc, *gc.tcount, lane, G);
+                wave_sync();
+                if (active && lane == 0) *gc.tcount = 0;
+                wave_sync();
+                append_local(listed, u, d.cand, &s_ncand);
+            }
+            __syncthreads();
+        }
+        nev = block_total(nev, s_red);
+        degs = block_total(degs, s_red);
+        if (tid == 0) {
+            s_acc[PT_MISS] = nev;
+            s_acc[PT_MISS_DEG] = degs;
+        }
+        // ---- minibucket prefixes, one wave per block range
+        if (mode) {
+            for (u32 p = tid / WAVE; p < d.k; p += BM / WAVE) {
+                const u64 base = (u64) p * d.width;
+                s64 carry = 0;
+                for (u32 c0 = 0; c0 < d.width; c0 += WAVE) {
+                    const u32 i = c0 + lane_id();
+                    const s64 v = i < d.width ? d.btot[base + i] : 0;
+                    const s64 inc = wave_inclusive_scan(v);
+                    if (i < d.width) {
+                        d.Q[base + i] = carry + inc - v;
+                        d.R[base + i] = carry + inc;
+                        d.btot[base + i] = 0;
+                    }
+                    carry += __shfl(inc, WAVE - 1, WAVE);
+                }
+                if (lane_id() == 0) {
+                    d.Tseg[p] = carry;
+                    if (p == d.k - 1) d.btot[(u64) d.k * d.width] = 0;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- afterburner / classify
+        const u32 nc = s_ncand;
+        if (mode == 0) {
+            u64 cdeg = 0, gh = 0;
+            afterburner_list<G>(d, stamp, d.cand, nc, d.moves_x, &s_nmoves, cdeg, gh);
+            cdeg = block_total(cdeg, s_red);
+            gh = block_total(gh, s_red);
+            if (tid == 0) {
+                s_acc[PT_CAND_DEG] = cdeg;
+                s_acc[PT_GHASH] = gh;
+            }
+        } else {
+            for (u32 i0 = 0; i0 < nc; i0 += BM) {
+                const u32 i = i0 + tid;
+                const bool valid = i < nc;
+                const u32 u = valid ? d.cand[i] : 0u;
+                bool sure = false, boundary = false;
+                u64 key = 0;
+                if (valid) {
+                    const u32 g = d.gid[u], ub = d.work[u];
+                    const s64 limit = d.bw[ub] - d.lmax;
+                    if (bucket_prefix(d, g, ub) < limit) {
+                        if (!strong && d.R[g] < limit) {
+                            sure = true;
+                            d.mark[u] = stamp;
+                            d.oldp[u] = ub;
+                        } else {
+                            boundary = true;
+                            key = ((u64) g << 32) | u;
+                        }
+                    }
+                }
+                append_local(sure, u, d.moves_x, &s_nmoves);
+                // rlist through a 32-bit LDS counter: entries are written by position
+                const u64 bal = __ballot(boundary);
+                if (bal) {
+                    const u32 first = (u32) __builtin_ctzll(bal);
+                    u32 base = 0;
+                    if (lane_id() == first) base = atomicAdd(&s_nrl, (u32) __popcll(bal));
+                    base = __shfl(base, (int) first, WAVE);
+                    if (boundary) d.rlist[base + (u32) __popcll(bal & lanemask_lt())] = key;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- resolve
+        const u32 L = s_nrl;
+        if (mode) {
+            const bool ok = resolve_body<BM_RCAP>(d, strong, stamp, L, rl, &s_nmoves);
+            if (!ok) {
+                if (tid == 0) {
+                    ctl->st[ST_MISS] = s_acc[PT_MISS];
+                    ctl->st[ST_MISS_DEG] = s_acc[PT_MISS_DEG];
+                    ctl->st[ST_CAND] = nc;
+                    ctl->st[ST_MOVES] = s_nmoves;
+                    ctl->st[ST_RLIST] = L;
+                    ctl->stalled = 1;
+                }
+                break;
+            }
+        }
+        __syncthreads();
+        // ---- apply
+        const u32 nm = s_nmoves;
+        s64 acc = 0;
+        u64 mdeg = 0, mh = 0;
+        for (u32 i0 = 0; i0 < nm; i0 += BM / G) {
+            const u32 i = i0 + gi;
+            apply_one<G>(d, i < nm ? d.moves_x[i] : 0u, i < nm, lane, stamp, d.trace != nullptr, acc, mdeg, mh);
+        }
+        const u64 a = block_total((u64) acc, s_red);
+        mdeg = block_total(mdeg, s_red);
+        mh = block_total(mh, s_red);
+        __syncthreads();
+        // ---- control (the barrier above completes the block weight updates)
+        s64 mx = INT64_MIN;
+        for (u32 b = tid; b < d.k; b += BM) mx = d.bw[b] > mx ? d.bw[b] : mx;
+        mx = wave_max(mx);
+        if (lane_id() == 0) ((s64 *) s_red)[tid / WAVE] = mx;
+        __syncthreads();
+        if (tid == 0) {
+            s64 m = INT64_MIN;
+            for (u32 w = 0; w < BM / WAVE; ++w) m = ((s64 *) s_red)[w] > m ? ((s64 *) s_red)[w] : m;
+            u64 t[PT_COUNT + 2];
+            for (u32 j = 0; j < PT_COUNT; ++j) t[j] = s_acc[j];
+            t[PT_DELTA] = a;
+            t[PT_MOVE_DEG] = mdeg;
+            t[PT_MHASH] = mh;
+            t[PT_COUNT] = nc;
+            t[PT_COUNT + 1] = nm;
+            ctl->st[ST_RLIST] = L;
+            control_step(d, m, t);
+        }
+        __syncthreads();
     }
 }
 
@@ -1415,6 +1751,9 @@ struct JetHost {
     JetDev d{};
     u32 G = 16, Gs = 16, sweep_threads = JB;
     bool big_resolve = false;
+    bool block_mode = false;
+    u32 bm_groups = 1;
+    size_t bm_lds = 0;
     size_t sweep_smem = 0, hub_smem = 0;
     s64 *H = nullptr, *roff = nullptr;
     u32 *over_flag = nullptr, *olist = nullptr, *olen = nullptr;
@@ -1439,6 +1778,11 @@ struct JetHost {
             launch(c, "jet_apply", 0.0, dim3(meta.n_hubs), dim3(JB), 0, apply_hub_kernel, d, (const u32 *) meta.hubs);
         HEIPA_DISPATCH_G(G, launch_apply);
         launch(c, "jet_control", 0.0, dim3(1), dim3(JB), 0, control_kernel, d);
+    }
+
+    template<int GG>
+    void launch_block(u32 max_iters) {
+        launch(c, "jet_block", 0.0, dim3(1), dim3(BM), bm_lds, jet_block_kernel<GG>, d, max_iters, bm_groups);
     }
 
     void iteration() {
@@ -1520,6 +1864,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.mark = bs.get<u32>(n);
     d.oldp = bs.get<u32>(n);
     d.gcache = bs.get<s64>(n);
+    d.icache = bs.get<s64>(n);
     d.gain1 = bs.get<s64>(n);
     d.lock = bs.get<u8>(n);
     d.bw = bs.get<s64>(k);
@@ -1573,6 +1918,14 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         d.n_ratios = rt.n;
     }
     h.big_resolve = c.max_lds >= 160 * 1024;
+    // single-workgroup form for small levels without hubs
+    {
+        const size_t cgb = conn_group_bytes(k);
+        h.bm_groups = (u32) std::max<size_t>(1, std::min<size_t>(BM / meta.G, BM_CONN_LDS / cgb));
+        h.bm_lds = std::max(bm_sweep_lds_bytes(k, h.bm_groups), resolve_lds_bytes(BM_RCAP));
+        h.block_mode = n <= BM_MAX_N && meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536) &&
+                       c.block_jet;
+    }
     d.rcap = std::min<u32>(c.resolve_cap, h.big_resolve ? RESOLVE_CAP : RESOLVE_CAP_SMALL);
     // lane-group geometry: G lanes per vertex; the sweep keeps one LDS
     // k-counter array per group, so wide k shrinks the groups per workgroup and,
@@ -1587,19 +1940,18 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         while (gpb > WAVE / Gs && gpb * cgb > SWEEP_LDS) gpb >>= 1;
         h.Gs = Gs;
         h.sweep_threads = gpb * Gs;
-        h.sweep_smem = cgb * gpb + 4ull * WAVE * (h.sweep_threads / WAVE) + 16 + 4ull * k;
+        h.sweep_smem = cgb * gpb + 4ull * SWEEP_PIECE + 16 + 4ull * k;
         h.hub_smem = cgb + 16 + 4ull * k;
         require(h.sweep_smem <= 65536 && h.hub_smem <= 65536, "k too large for the LDS block-connectivity counters");
     }
-    // list regions: sweep workgroup r visits at most `waves * ceil(tiles / (nreg * waves))`
-    // tiles of 64 vertices; its post workgroup may add its share of the extra
-    // candidate list (hubs) to its move region
+    // list regions: sweep workgroup r owns vertices [r * cap, (r + 1) * cap),
+    // cap a multiple of the workgroup size; its post workgroup may add its
+    // share of the extra candidate list (hubs) to its move region
     {
-        const u32 waves = h.sweep_threads / WAVE;
-        const u64 tiles = div_up(n, WAVE);
-        d.nreg = grid_for(c, tiles, waves, 8);
-        const u64 per = (tiles + (u64) d.nreg * waves - 1) / ((u64) d.nreg * waves);
-        d.cap = (u32) (per * waves * WAVE);
+        const u64 per_block = h.sweep_threads;
+        d.nreg = grid_for(c, n, (u32) per_block, 8);
+        d.cap = (u32) (div_up(n, (u64) d.nreg * per_block) * per_block);
+        d.nreg = div_up(n, d.cap);
         d.cap_mv = d.cap + div_up(meta.n_hubs, d.nreg);
         d.cand = bs.get<u32>((u64) d.nreg * d.cap);
         d.moves = bs.get<u32>((u64) d.nreg * d.cap_mv);
@@ -1611,6 +1963,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     }
     copy_d2d(c, d.work, part, 4ull * n);
     fill<u32>(c, d.dcache, n, NULL_PART);
+    fill<s64>(c, d.icache, n, -1);
     fill<u8>(c, d.lock, n, (u8) 0);
     fill<u32>(c, d.mark, n, 0u);
     fill<u32>(c, h.over_flag, n, 0u);
@@ -1623,11 +1976,15 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     char *slot[2] = {(char *) c.pinned + 65536, (char *) c.pinned + 65536 + slot_bytes};
     hipEvent_t ev[2] = {c.get_event(), c.get_event()};
     u32 B = 2, consumed = 0;
-    u64 enqueued = 0;
+    u64 submits = 0;
+    const u32 bm_iters = tracing ? TRACE_RING / 2 : (1u << 20);
     auto submit = [&](int sidx) {
-        for (u32 i = 0; i < B; ++i) h.iteration();
-        enqueued += B;
-        if (enqueued > (u64) 1 << 22) throw Error(HEIPA_ERR_DEVICE, "jet_refine: iteration guard exceeded");
+        if (h.block_mode) {
+            HEIPA_DISPATCH_G(h.G, h.template launch_block, bm_iters);
+        } else {
+            for (u32 i = 0; i < B; ++i) h.iteration();
+        }
+        if (++submits > (u64) 1 << 20) throw Error(HEIPA_ERR_DEVICE, "jet_refine: iteration guard exceeded");
         copy_d2h(c, slot[sidx], ctl_mem, snap_bytes);
         HIP_CHECK(hipEventRecord(ev[sidx], c.stream));
     };
@@ -1657,11 +2014,15 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     c.event_pool.push_back(ev[0]);
     c.event_pool.push_back(ev[1]);
     if (s.save) copy_d2d(c, part, d.work, 4ull * n);
-    prof_bytes(c, "jet_sweep", s.bytes[0]);
-    prof_bytes(c, "jet_buckets", s.bytes[1]);
-    prof_bytes(c, "jet_post", s.bytes[2]);
-    prof_bytes(c, "jet_resolve", s.bytes[3]);
-    prof_bytes(c, "jet_apply", s.bytes[4]);
+    if (h.block_mode) {
+        prof_bytes(c, "jet_block", s.bytes[0] + s.bytes[1] + s.bytes[2] + s.bytes[3] + s.bytes[4]);
+    } else {
+        prof_bytes(c, "jet_sweep", s.bytes[0]);
+        prof_bytes(c, "jet_buckets", s.bytes[1]);
+        prof_bytes(c, "jet_post", s.bytes[2]);
+        prof_bytes(c, "jet_resolve", s.bytes[3]);
+        prof_bytes(c, "jet_apply", s.bytes[4]);
+    }
     c.sync();
     res.cut = s.best_cut;
     res.maxw = s.best_maxw;

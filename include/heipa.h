@@ -153,6 +153,9 @@ double heipa_adaptive_imbalance(double global_imbalance, int64_t global_g_weight
  * allows; 0 forces the device-wide path).  Results do not depend on it; tests use it to cover
  * both paths. */
 #define HEIPA_OPT_RESOLVE_CAP 1
+/* HEIPA_OPT_BLOCK_JET: 1 (default) lets Jet run levels of at most 1024 vertices without hubs in
+ * one workgroup; 0 forces the multi-launch form everywhere.  Results do not depend on it. */
+#define HEIPA_OPT_BLOCK_JET 2
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */

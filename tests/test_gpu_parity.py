@@ -119,17 +119,32 @@ def test_jet(ctx, graph, ultra, level, skew, k):
     assert orc.edge_cut(g, gp) == gcut
 
 
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.0, 8), (True, 0, 0.0, 8), (False, 0, 0.6, 8),
+                                                (True, 2, 0.9, 8), (False, 0, 0.3, 3), (False, 1, 0.7, 7)])
+def test_jet_multi_launch(ctx, graph, ultra, level, skew, k):
+    """Small graphs run Jet in one workgroup by default; this forces the
+    multi-launch form on them."""
+    ctx.set_option(heipa.OPT_BLOCK_JET, 0)
+    try:
+        test_jet(ctx, graph, ultra, level, skew, k)
+    finally:
+        ctx.set_option(heipa.OPT_BLOCK_JET, 1)
+
+
+@pytest.mark.parametrize("block", [1, 0])
 @pytest.mark.parametrize("cap", [0, 16])
 @pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 1, 0.7, 7)])
-def test_jet_global_resolve(ctx, graph, cap, ultra, level, skew, k):
+def test_jet_global_resolve(ctx, graph, cap, block, ultra, level, skew, k):
     """Rebalance iterations whose resolve list exceeds the LDS capacity take the
-    device-wide radix-sort path; a capacity of 0 sends every rebalance there, 16
-    mixes both paths within one refinement."""
+    device-wide radix-sort path (from either Jet form); a capacity of 0 sends
+    every rebalance there, 16 mixes both paths within one refinement."""
     ctx.set_option(heipa.OPT_RESOLVE_CAP, cap)
+    ctx.set_option(heipa.OPT_BLOCK_JET, block)
     try:
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
         ctx.set_option(heipa.OPT_RESOLVE_CAP, 1 << 32)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 1)
 
 
 @pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False), (3, False), (7, True)])
