@@ -23,6 +23,10 @@ WORKLOADS = {
     "rgg20-ultra-k64": dict(kind="rgg", n=1 << 20, deg=12.0, seed=1, wmod=10, k=64, ultra=True),
     "grid512-default-k32": dict(kind="grid", rows=512, cols=512, k=32, ultra=False),
     "rgg16-ultra-k64": dict(kind="rgg", n=1 << 16, deg=12.0, seed=1, wmod=10, k=64, ultra=True),
+    # batched form: independent small graphs through a worker pool
+    "batched": dict(kind="batched"),
+    # served form: Poisson arrivals of partition and mapping requests
+    "served": dict(kind="served"),
 }
 
 
@@ -42,11 +46,21 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-all", action="store_true", help="print the per-kernel event profile to stderr")
+    ap.add_argument("--slots", type=int, default=16, help="batched/served: pool workers")
+    ap.add_argument("--batch-count", type=int, default=4096, help="batched: number of graphs")
+    ap.add_argument("--plain-sample", type=int, default=256, help="batched: graphs timed through the plain path")
+    ap.add_argument("--requests", type=int, default=240, help="served: number of requests")
+    ap.add_argument("--served-nmax", type=float, default=1e6, help="served: largest request n")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path does not shard "
               f"across GPUs in this build)", file=sys.stderr)
         sys.exit(2)
+    if WORKLOADS[args.workload]["kind"] in ("batched", "served"):
+        # one hardware queue per worker stream, so the workers' kernels overlap
+        # (set before the HIP runtime starts)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(max(4, min(args.slots, 16)))
+        return run_pool_workload(args)
 
     import numpy as np
     import torch
@@ -105,8 +119,21 @@ def main():
                 "launches": s["launches"], "avg_us": round(s["ms"] * 1000.0 / max(s["launches"], 1), 3),
                 "alg_bytes_per_launch": round(s["bytes"] / max(s["launches"], 1), 1)}
 
+    # the same partition from host CSR to host partition array (upload, partition,
+    # download): the PCIe-inclusive rate, reported beside `value`, never as it
+    torch.cuda.synchronize()
+    th0 = time.perf_counter()
+    dg_h = heipa.DeviceGraph(g, "cuda:0")
+    part_h, _, _ = ctx.partition(dg_h, k, 0.03, args.seed, ultra)
+    host_part = heipa.to_numpy_u32(part_h)
+    th1 = time.perf_counter()
+    del dg_h, part_h
+    pcie = {"ms_per_step": round((th1 - th0) * 1000.0, 3), "value": round(m_und / (th1 - th0), 1),
+            "includes": "H2D upload of the CSR, partition, D2H download of the partition (one step)"}
+
     # quality beside the number
     p = heipa.to_numpy_u32(part)
+    assert np.array_equal(p, host_part)
     vw = np.ones(g.n, np.int64) if g.vwgt is None else g.vwgt.astype(np.int64)
     bw = np.bincount(p, weights=vw, minlength=k)
     quality = {"cut": int(stats.cut), "maxw": int(stats.maxw), "lmax": int(stats.lmax),
@@ -147,7 +174,170 @@ def main():
         "quality": quality,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
     }
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+def _gen_batched(count):
+    """SURVEY config 4: graph i uses seed 1000+i; RGG (avg degree 8) or R-MAT
+    (edge factor 8, scale ceil(log2 n), ids >= n rejected) with probability 0.5
+    each; n log-uniform in [1024, 65536]; edge weights 1 + hash mod 10; k from
+    {2,4,8,16,32}; eps 0.03, default config."""
+    import math
+    import numpy as np
+    import heipa
+    rng = np.random.default_rng(1)
+    out = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(1024), math.log(65536)))))
+        rgg = rng.random() < 0.5
+        k = int(rng.choice([2, 4, 8, 16, 32]))
+        if rgg:
+            g = heipa.gen_rgg(n, 8.0, 1000 + i, 10)
+        else:
+            g = heipa.gen_rmat(max(1, math.ceil(math.log2(n))), n, 8.0, 0.57, 0.19, 0.19, 1000 + i, 10, False)
+        out.append((g, dict(k=k), "rgg" if rgg else "rmat"))
+    return out
+
+
+def _gen_served(count, nmax):
+    """SURVEY config 5 (request mix): 70% partition (k uniform in 8..256), 30%
+    mapping (hierarchy 4:8:6 or 2:4:8); grid / RGG (avg degree 12) / R-MAT (edge
+    factor 16) with probability 1/3 each; n log-uniform in [1e4, nmax]; edge
+    weights 1 + hash mod 100."""
+    import math
+    import numpy as np
+    import heipa
+    rng = np.random.default_rng(1)
+    out = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(1e4), math.log(nmax)))))
+        t = int(rng.integers(3))
+        if t == 0:
+            side = max(2, int(round(math.sqrt(n))))
+            g = heipa.gen_grid(side, side)
+            kind = "grid"
+        elif t == 1:
+            g = heipa.gen_rgg(n, 12.0, 2000 + i, 100)
+            kind = "rgg"
+        else:
+            g = heipa.gen_rmat(max(1, math.ceil(math.log2(n))), n, 16.0, 0.57, 0.19, 0.19, 2000 + i, 100, False)
+            kind = "rmat"
+        if rng.random() < 0.7:
+            spec = dict(k=int(rng.integers(8, 257)))
+        else:
+            spec = dict(hier=[4, 8, 6] if rng.random() < 0.5 else [2, 4, 8])
+        out.append((g, spec, kind))
+    return out
+
+
+def _run_alone(ctx, dg, spec):
+    import heipa
+    if "hier" in spec:
+        return heipa.to_numpy_u32(ctx.hm_partition(dg, spec["hier"], 0.03, 1, False))
+    part, _, _ = ctx.partition(dg, spec["k"], 0.03, 1, False)
+    return heipa.to_numpy_u32(part)
+
+
+def run_pool_workload(args):
+    import numpy as np
+    import torch
+    import heipa
+
+    served = args.workload == "served"
+    tg = time.perf_counter()
+    reqs = _gen_served(args.requests, args.served_nmax) if served else _gen_batched(args.batch_count)
+    dgs = [heipa.DeviceGraph(g, "cuda:0") for g, _, _ in reqs]
+    gen_s = time.perf_counter() - tg
+    edges = [g.m // 2 for g, _, _ in reqs]
+    probe = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **spec) for dg, (_, spec, _) in zip(dgs, reqs)]
+    pool0 = heipa.Pool(1, 1 << 20)
+    est = [pool0.estimate(j) for j in probe]
+    pool0.close()
+    del probe
+    arena = int(max(est) * 1.05) + (32 << 20)
+    ctx = heipa.Context(0, arena)
+
+    # plain path: requests one after another on one context (the drop-in)
+    sample = list(range(min(args.plain_sample if not served else 24, len(reqs))))
+    for i in sample[:2]:
+        _run_alone(ctx, dgs[i], reqs[i][1])
+    alone = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    service = []
+    for i in sample:
+        ts = time.perf_counter()
+        alone[i] = _run_alone(ctx, dgs[i], reqs[i][1])
+        service.append(time.perf_counter() - ts)
+    plain_s = time.perf_counter() - t0
+    plain_rate = sum(edges[i] for i in sample) / plain_s
+
+    pool = heipa.Pool(args.slots, arena)
+    warm = [heipa.Pool.job(dgs[i], imbalance=0.03, seed=1, **reqs[i][1]) for i in range(min(2 * args.slots, len(reqs)))]
+    pool.run(warm)
+    del warm
+    jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **spec) for dg, (_, spec, _) in zip(dgs, reqs)]
+    result = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "int32/int64", "data": "synthetic (seeded generators)", "steps": 1, "warmup": 1}
+    if not served:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pool.run(jobs)
+        wall = time.perf_counter() - t0
+        value = sum(edges) / wall
+        result.update({"metric": "input edges/sec on 1 MI355X, batched form", "value": round(value, 1),
+                       "unit": "input edges/s", "ms_per_step": round(wall * 1000.0, 3),
+                       "config": {"workload": "batched", "graphs": len(reqs), "undirected_edges": sum(edges),
+                                  "slots": args.slots, "arena_bytes_per_slot": arena,
+                                  "mix": {"rgg": sum(1 for r in reqs if r[2] == "rgg"),
+                                          "rmat": sum(1 for r in reqs if r[2] == "rmat")}}})
+    else:
+        capacity = len(sample) / plain_s  # requests/s of the plain path on the sample
+        lam = 0.8 * capacity
+        rng = np.random.default_rng(1)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+        t0 = time.perf_counter()
+        for j, a in zip(jobs, arrivals):
+            dt = a - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            pool.submit(j)
+        for j in jobs:
+            pool.wait(j)
+        lat = np.array([j.t_done - j.t_submit for j in jobs])
+        makespan = (max(j.t_done for j in jobs) - min(j.t_submit for j in jobs)) / 1000.0
+        value = sum(edges) / makespan
+        result.update({"metric": "input edges/sec on 1 MI355X, served form", "value": round(value, 1),
+                       "unit": "input edges/s", "ms_per_step": round(makespan * 1000.0, 3),
+                       "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                      "p99": round(float(np.percentile(lat, 99)), 3),
+                                      "max": round(float(lat.max()), 3)},
+                       "config": {"workload": "served", "requests": len(reqs), "undirected_edges": sum(edges),
+                                  "offered_rate_per_s": round(lam, 3),
+                                  "plain_capacity_per_s": round(capacity, 3), "n_max": args.served_nmax,
+                                  "slots": args.slots, "arena_bytes_per_slot": arena,
+                                  "partition_requests": sum(1 for r in reqs if "k" in r[1]),
+                                  "mapping_requests": sum(1 for r in reqs if "hier" in r[1])}})
+    pool.close()
+    bad = [j for j in jobs if j.status != heipa.OK]
+    same = sum(1 for i in sample if np.array_equal(heipa.to_numpy_u32(jobs[i]._part)[:dgs[i].n], alone[i]))
+    balanced = 0
+    for (g, spec, _), j, dg in zip(reqs, jobs, dgs):
+        kk = spec["k"] if "k" in spec else int(np.prod(spec["hier"]))
+        p = heipa.to_numpy_u32(j._part)[:dg.n]
+        lmax = int(np.ceil(1.03 * g.g_weight / kk)) if "k" in spec else None
+        w = np.bincount(p, minlength=kk)
+        balanced += int(lmax is None or w.max() <= lmax)
+    result["plain"] = {"value": round(plain_rate, 1), "unit": "input edges/s", "requests": len(sample),
+                       "seconds": round(plain_s, 3)}
+    result["speedup_vs_plain"] = round(result["value"] / plain_rate, 3)
+    result["quality"] = {"failed": len(bad), "identical_to_alone": f"{same}/{len(sample)}",
+                         "partition_jobs_balanced": balanced,
+                         "sum_cut": int(sum(j.cut for j in jobs))}
+    result["generation_s"] = round(gen_s, 2)
     print(json.dumps(result), flush=True)
     ctx.close()
 

@@ -1,0 +1,213 @@
+// Batched and served forms: a pool of workers on one device.
+//
+// Each worker owns a heipa_ctx (its own HIP stream, arena and pinned staging)
+// and a host thread.  Jobs wait in one FIFO queue; the head job starts when a
+// worker is free and the device-memory estimates of the running jobs plus its
+// own stay within the pool's budget (admission by estimate, so requests are
+// admitted as workers and memory free up).  A job runs exactly the
+// single-graph entry point (heipa_partition / heipa_hm_partition) on its
+// worker's context: its result is the same alone and in the mix, and the
+// kernels of jobs on different workers overlap on the device.
+#include "../../../include/heipa.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+struct heipa_pool {
+    int device = 0;
+    uint64_t arena_bytes = 0, budget = 0;
+    std::vector<heipa_ctx *> ctxs;
+    std::vector<std::thread> threads;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<heipa_job *> queue;
+    uint64_t inflight = 0;
+    bool stop = false;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+// Measured arena peaks of the single-graph path stay below this: the
+// hierarchy's graphs and mappings (a geometric series over levels, at most
+// twice the input), two partition arrays, per-level matching and contraction
+// scratch (sort buffers over the edges) and the Jet state (about 14 arrays
+// per vertex, minibucket arrays of 50 k sections entries, list regions).
+uint64_t estimate(const heipa_job &j) {
+    const uint64_t n = j.graph.n, m = j.graph.m;
+    uint64_t k = j.k;
+    if (j.kind == HEIPA_JOB_HM) {
+        k = 1;
+        for (uint32_t i = 0; i < j.l && i < 8; ++i) k = std::max<uint64_t>(k, j.hier[i]);
+    }
+    const uint64_t sections = std::max<uint64_t>(1, std::min<uint64_t>(128, n / (4096 * std::max<uint64_t>(k, 1)) + 1));
+    const uint64_t bucket_bytes = 50ull * k * sections * 32;
+    uint64_t bytes = 256 * n + 56 * m + bucket_bytes + (64ull << 20);
+    if (j.kind == HEIPA_JOB_HM) bytes += 16 * n + 16 * m;  // child graphs of the recursion
+    return bytes;
+}
+
+void run_job(heipa_ctx *ctx, heipa_job &j) {
+    if (j.kind == HEIPA_JOB_PARTITION) {
+        heipa_solve_stats st{};
+        j.status = heipa_partition(ctx, &j.graph, j.k, j.imbalance, j.seed, j.ultra, j.part, &st, nullptr, 0);
+        if (j.status == HEIPA_OK) {
+            j.cut = st.cut;
+            j.maxw = st.maxw;
+        }
+        return;
+    }
+    if (j.kind != HEIPA_JOB_HM || j.l == 0 || j.l > 8) {
+        j.status = HEIPA_ERR_INVALID;
+        return;
+    }
+    j.status = heipa_hm_partition(ctx, &j.graph, j.hier, j.l, j.imbalance, j.seed, j.ultra, j.part);
+    if (j.status != HEIPA_OK) return;
+    uint32_t k = 1;
+    for (uint32_t i = 0; i < j.l; ++i) k *= j.hier[i];
+    j.status = heipa_edge_cut(ctx, &j.graph, j.part, &j.cut);
+    if (j.status != HEIPA_OK) return;
+    std::vector<int64_t> bw(k);
+    j.status = heipa_block_weights(ctx, &j.graph, j.part, k, bw.data(), &j.maxw);
+}
+
+void worker(heipa_pool *p, uint32_t slot) {
+    for (;;) {
+        heipa_job *job = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(p->m);
+            p->cv_work.wait(lk, [&] {
+                return (p->stop && p->queue.empty()) ||
+                       (!p->queue.empty() && p->inflight + p->queue.front()->estimate <= p->budget);
+            });
+            if (p->queue.empty()) return;  // stopping
+            job = p->queue.front();
+            p->queue.pop_front();
+            p->inflight += job->estimate;
+            job->state = 2;
+            job->slot = (int32_t) slot;
+            job->t_start = p->now();
+        }
+        run_job(p->ctxs[slot], *job);
+        {
+            std::lock_guard<std::mutex> lk(p->m);
+            p->inflight -= job->estimate;
+            job->t_done = p->now();
+            job->state = 3;
+        }
+        p->cv_work.notify_all();
+        p->cv_done.notify_all();
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+uint64_t heipa_job_estimate(const heipa_job *job) { return job ? estimate(*job) : 0; }
+
+int heipa_pool_create(int device, uint32_t slots, uint64_t arena_bytes, uint64_t memory_budget, heipa_pool **out) {
+    if (!out || slots == 0 || arena_bytes == 0) return HEIPA_ERR_INVALID;
+    *out = nullptr;
+    heipa_pool *p = new (std::nothrow) heipa_pool();
+    if (!p) return HEIPA_ERR_OOM;
+    p->device = device;
+    p->arena_bytes = arena_bytes;
+    p->budget = memory_budget ? memory_budget : arena_bytes * slots;
+    for (uint32_t s = 0; s < slots; ++s) {
+        heipa_ctx *c = nullptr;
+        const int rc = heipa_ctx_create(device, arena_bytes, &c);
+        if (rc != HEIPA_OK) {
+            for (heipa_ctx *x: p->ctxs) heipa_ctx_destroy(x);
+            delete p;
+            return rc;
+        }
+        p->ctxs.push_back(c);
+    }
+    try {
+        for (uint32_t s = 0; s < slots; ++s) p->threads.emplace_back(worker, p, s);
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(p->m);
+            p->stop = true;
+        }
+        p->cv_work.notify_all();
+        for (auto &t: p->threads) t.join();
+        for (heipa_ctx *x: p->ctxs) heipa_ctx_destroy(x);
+        delete p;
+        return HEIPA_ERR_OOM;
+    }
+    *out = p;
+    return HEIPA_OK;
+}
+
+int heipa_pool_destroy(heipa_pool *p) {
+    if (!p) return HEIPA_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(p->m);
+        p->stop = true;
+    }
+    p->cv_work.notify_all();
+    for (auto &t: p->threads) t.join();
+    for (heipa_ctx *x: p->ctxs) heipa_ctx_destroy(x);
+    delete p;
+    return HEIPA_OK;
+}
+
+int heipa_pool_submit(heipa_pool *p, heipa_job *job) {
+    if (!p || !job) return HEIPA_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(p->m);
+    if (p->stop) return HEIPA_ERR_INVALID;
+    job->slot = -1;
+    job->cut = job->maxw = 0;
+    job->t_submit = p->now();
+    job->t_start = job->t_done = 0.0;
+    job->estimate = estimate(*job);
+    if (job->estimate > p->arena_bytes || job->estimate > p->budget) {
+        job->status = HEIPA_ERR_OOM;
+        job->state = 3;
+        job->t_done = job->t_submit;
+        p->cv_done.notify_all();
+        return HEIPA_OK;
+    }
+    job->status = HEIPA_OK;
+    job->state = 1;
+    p->queue.push_back(job);
+    p->cv_work.notify_all();
+    return HEIPA_OK;
+}
+
+int heipa_pool_wait(heipa_pool *p, heipa_job *job, double timeout_ms) {
+    if (!p || !job) return 0;
+    std::unique_lock<std::mutex> lk(p->m);
+    auto done = [&] { return job->state == 3; };
+    if (timeout_ms < 0) {
+        p->cv_done.wait(lk, done);
+        return 1;
+    }
+    return p->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done) ? 1 : 0;
+}
+
+int heipa_pool_run(heipa_pool *p, heipa_job *jobs, uint32_t count) {
+    if (!p || (count && !jobs)) return HEIPA_ERR_INVALID;
+    for (uint32_t i = 0; i < count; ++i) {
+        const int rc = heipa_pool_submit(p, &jobs[i]);
+        if (rc != HEIPA_OK) return rc;
+    }
+    for (uint32_t i = 0; i < count; ++i) heipa_pool_wait(p, &jobs[i], -1.0);
+    return HEIPA_OK;
+}
+
+double heipa_pool_clock(const heipa_pool *p) { return p ? p->now() : 0.0; }
+
+} // extern "C"

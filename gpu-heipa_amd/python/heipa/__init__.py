@@ -55,6 +55,20 @@ class JetTrace(ctypes.Structure):
                 ("moves_hash", ctypes.c_uint64), ("gain_hash", ctypes.c_uint64)]
 
 
+JOB_PARTITION, JOB_HM = 0, 1
+
+
+class Job(ctypes.Structure):
+    """heipa_job: one partition or HM request for a Pool."""
+    _fields_ = [("graph", Graph), ("kind", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("hier", ctypes.c_uint32 * 8), ("l", ctypes.c_uint32), ("ultra", ctypes.c_int32),
+                ("imbalance", ctypes.c_double), ("seed", ctypes.c_uint64), ("part", ctypes.c_void_p),
+                ("status", ctypes.c_int32), ("slot", ctypes.c_int32), ("cut", ctypes.c_int64),
+                ("maxw", ctypes.c_int64), ("estimate", ctypes.c_uint64), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double), ("state", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
 class LevelTrace(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint32), ("m", ctypes.c_uint32), ("nc", ctypes.c_uint32), ("pad", ctypes.c_uint32),
                 ("mapping_hash", ctypes.c_uint64), ("cut", ctypes.c_int64), ("maxw", ctypes.c_int64)]
@@ -110,6 +124,13 @@ def lib():
             "heipa_hm_partition": (c_int, [vp, GP, u32p, c_u32, c_dbl, c_u64, c_int, vp]),
             "heipa_adaptive_imbalance": (c_dbl, [c_dbl, ctypes.c_int64, c_u32, ctypes.c_int64, c_u32, c_u64]),
             "heipa_ctx_set_option": (c_int, [vp, c_int, c_u64]),
+            "heipa_pool_create": (c_int, [c_int, c_u32, c_u64, c_u64, ctypes.POINTER(vp)]),
+            "heipa_pool_destroy": (c_int, [vp]),
+            "heipa_job_estimate": (c_u64, [ctypes.POINTER(Job)]),
+            "heipa_pool_run": (c_int, [vp, ctypes.POINTER(Job), c_u32]),
+            "heipa_pool_submit": (c_int, [vp, ctypes.POINTER(Job)]),
+            "heipa_pool_wait": (c_int, [vp, ctypes.POINTER(Job), c_dbl]),
+            "heipa_pool_clock": (c_dbl, [vp]),
             "heipa_prof_enable": (c_int, [vp, c_int]),
             "heipa_prof_reset": (c_int, [vp]),
             "heipa_prof_filter": (c_int, [vp, ctypes.c_char_p]),
@@ -463,6 +484,80 @@ class Context:
                                                                 ctypes.byref(by)))
             out[buf.value.decode()] = {"launches": ln.value, "ms": ms.value, "bytes": by.value}
         return out
+
+
+class Pool:
+    """heipa_pool: `slots` workers, each with its own context (stream, arena of
+    arena_bytes), running jobs from one FIFO queue under a device-memory budget."""
+
+    def __init__(self, slots, arena_bytes, budget=0, device=0):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        rc = self.L.heipa_pool_create(device, slots, arena_bytes, budget, ctypes.byref(self.h))
+        if rc != OK:
+            raise HeipaError("heipa_pool_create", rc)
+        self._keep = {}
+
+    @staticmethod
+    def job(g: "DeviceGraph", k=0, hier=None, imbalance=0.03, seed=1, ultra=False, part=None):
+        """A job for g (partition into k blocks, or HM with hierarchy `hier`); the
+        device partition tensor is allocated unless given."""
+        torch = _torch()
+        j = Job()
+        j.graph = g.desc
+        if hier:
+            j.kind, j.l = JOB_HM, len(hier)
+            for i, a in enumerate(hier):
+                j.hier[i] = a
+        else:
+            j.kind, j.k = JOB_PARTITION, k
+        j.ultra, j.imbalance, j.seed = int(ultra), imbalance, seed
+        if part is None:
+            part = torch.empty(max(g.n, 1), dtype=torch.int32, device=g.xadj.device)
+        j.part = part.data_ptr()
+        j._part, j._graph = part, g  # keep the tensors alive with the job
+        return j
+
+    def estimate(self, job):
+        return self.L.heipa_job_estimate(ctypes.byref(job))
+
+    def run(self, jobs):
+        """Batched form: all jobs, returns when every one is done."""
+        _torch().cuda.synchronize()
+        arr = (Job * len(jobs))(*jobs)
+        rc = self.L.heipa_pool_run(self.h, arr, len(jobs))
+        if rc != OK:
+            raise HeipaError("heipa_pool_run", rc)
+        for i, j in enumerate(jobs):  # copy the results back into the caller's objects
+            ctypes.memmove(ctypes.addressof(j), ctypes.addressof(arr[i]), ctypes.sizeof(Job))
+        return jobs
+
+    def submit(self, job):
+        """Served form: queue one job (it must stay alive until done)."""
+        self._keep[id(job)] = job
+        rc = self.L.heipa_pool_submit(self.h, ctypes.byref(job))
+        if rc != OK:
+            raise HeipaError("heipa_pool_submit", rc)
+
+    def wait(self, job, timeout_ms=-1.0):
+        done = self.L.heipa_pool_wait(self.h, ctypes.byref(job), float(timeout_ms)) == 1
+        if done:
+            self._keep.pop(id(job), None)
+        return done
+
+    def clock(self):
+        return self.L.heipa_pool_clock(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.heipa_pool_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def to_numpy_u32(t):

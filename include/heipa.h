@@ -147,6 +147,50 @@ int heipa_hm_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hie
 double heipa_adaptive_imbalance(double global_imbalance, int64_t global_g_weight, uint32_t global_k,
                                 int64_t local_g_weight, uint32_t local_k_rem, uint64_t depth);
 
+/* ---- batched and served forms.  No counterpart in the reference: its apps solve one graph per process.
+ * A pool of `slots` workers on one device; each worker owns a context (HIP stream, arena of
+ * arena_bytes, pinned staging) and a host thread.  Jobs wait in one FIFO queue and start when a worker
+ * is free and the device-memory estimates of the running jobs plus the next one stay within
+ * memory_budget.  A job runs exactly heipa_partition or heipa_hm_partition on its worker's context, so
+ * its result does not depend on the other jobs; jobs on different workers overlap on the device (their
+ * streams map to separate hardware queues up to the runtime's GPU_MAX_HW_QUEUES). */
+typedef struct heipa_pool heipa_pool;
+#define HEIPA_JOB_PARTITION 0 /* Solver(config).solve: k blocks */
+#define HEIPA_JOB_HM 1        /* ProMapSolver::solve_multisection: hierarchy hier[0..l) */
+typedef struct {
+    /* inputs */
+    heipa_graph graph;        /* device arrays; valid until the job is done */
+    uint32_t kind;            /* HEIPA_JOB_PARTITION or HEIPA_JOB_HM */
+    uint32_t k;               /* partition jobs */
+    uint32_t hier[8];         /* HM jobs: a_1..a_l */
+    uint32_t l;
+    int32_t ultra;
+    double imbalance;
+    uint64_t seed;
+    uint32_t *part;           /* device, graph.n entries */
+    /* outputs (valid once done) */
+    int32_t status;           /* HEIPA_OK or the error code of the run */
+    int32_t slot;             /* worker that ran the job, -1 if it never ran */
+    int64_t cut, maxw;        /* edge cut and largest block weight of the result */
+    uint64_t estimate;        /* device bytes the job was admitted with */
+    double t_submit, t_start, t_done; /* ms on the pool's clock (heipa_pool_clock) */
+    uint32_t state;           /* 0 new, 1 queued, 2 running, 3 done (read through heipa_pool_wait) */
+    uint32_t reserved;
+} heipa_job;
+int heipa_pool_create(int device, uint32_t slots, uint64_t arena_bytes, uint64_t memory_budget, heipa_pool **out);
+int heipa_pool_destroy(heipa_pool *pool);  /* waits for queued and running jobs */
+/* device bytes a job needs in a worker's arena (an upper estimate from n, m and the number of blocks) */
+uint64_t heipa_job_estimate(const heipa_job *job);
+/* batched form: runs all jobs and returns when every one is done (HEIPA_OK even if some jobs failed;
+ * see each job's status) */
+int heipa_pool_run(heipa_pool *pool, heipa_job *jobs, uint32_t count);
+/* served form: queues a job and returns at once; a job whose estimate exceeds the arena or the budget
+ * is done at once with HEIPA_ERR_OOM */
+int heipa_pool_submit(heipa_pool *pool, heipa_job *job);
+/* waits until the job is done or timeout_ms passes (negative: no limit); 1 done, 0 not yet */
+int heipa_pool_wait(heipa_pool *pool, heipa_job *job, double timeout_ms);
+double heipa_pool_clock(const heipa_pool *pool);
+
 /* ---- context options.  No counterpart in the reference (its sizes are compile-time).
  * HEIPA_OPT_RESOLVE_CAP: most rebalance resolve entries one workgroup sorts in LDS before an
  * iteration takes the device-wide radix-sort path (default and maximum: what the device's LDS

@@ -1,0 +1,97 @@
+"""Batched and served forms (heipa_pool): every job's result equals the same
+request run alone on a single context (bit-exact partition arrays), under
+concurrency, arrival gaps and memory-budget admission."""
+import time
+
+import numpy as np
+import pytest
+
+import heipa
+from graphs import star_plus_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    g = [heipa.gen_grid(40, 50), heipa.gen_rgg(3000, 8.0, 3, 10), heipa.gen_rmat(11, 1500, 8.0, 0.57, 0.19, 0.19, 4, 10, False),
+         heipa.gen_rgg(5000, 12.0, 9, 100), star_plus_grid(), heipa.gen_grid(17, 90),
+         heipa.gen_rmat(12, 0, 16.0, 0.57, 0.19, 0.19, 8, 100, True), heipa.gen_rgg(2500, 8.0, 11, 0)]
+    specs = [dict(k=8), dict(k=4, ultra=True), dict(k=16), dict(hier=[2, 4, 3]), dict(k=3),
+             dict(k=32), dict(hier=[4, 8, 6]), dict(k=2)]
+    return [(heipa.DeviceGraph(x, "cuda:0"), s) for x, s in zip(g, specs)]
+
+
+def _alone(ctx, dg, s):
+    if "hier" in s:
+        return heipa.to_numpy_u32(ctx.hm_partition(dg, s["hier"], 0.03, 1, s.get("ultra", False)))
+    part, _, _ = ctx.partition(dg, s["k"], 0.03, 1, s.get("ultra", False))
+    return heipa.to_numpy_u32(part)
+
+
+def test_pool_batched_equals_alone(ctx):
+    reqs = _requests()
+    pool = heipa.Pool(3, 512 << 20)
+    try:
+        jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs]
+        pool.run(jobs)
+    finally:
+        pool.close()
+    slots = set()
+    for (dg, s), j in zip(reqs, jobs):
+        assert j.status == heipa.OK and j.state == 3, (s, j.status)
+        assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
+        assert j.cut == ctx.edge_cut(dg, j._part)
+        slots.add(j.slot)
+    assert len(slots) > 1  # the work was spread over workers
+
+
+def test_pool_served_equals_alone(ctx):
+    reqs = _requests()
+    pool = heipa.Pool(4, 512 << 20)
+    try:
+        jobs = []
+        for i, (dg, s) in enumerate(reqs):
+            j = heipa.Pool.job(dg, imbalance=0.03, seed=1, **s)
+            pool.submit(j)
+            jobs.append(j)
+            time.sleep(0.002 * (i % 3))
+        for j in jobs:
+            assert pool.wait(j, 60000.0)
+    finally:
+        pool.close()
+    for (dg, s), j in zip(reqs, jobs):
+        assert j.status == heipa.OK
+        assert j.t_submit <= j.t_start <= j.t_done
+        assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
+
+
+def test_pool_admission_budget_serialises(ctx):
+    """A budget that fits one job at a time admits them one by one."""
+    reqs = _requests()[:4]
+    probe = heipa.Pool(1, 512 << 20)
+    jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs]
+    budget = max(probe.estimate(j) for j in jobs)
+    probe.close()
+    pool = heipa.Pool(4, 512 << 20, budget=budget)
+    try:
+        pool.run(jobs)
+    finally:
+        pool.close()
+    spans = sorted((j.t_start, j.t_done) for j in jobs)
+    for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+        assert b0 >= a1  # no two jobs ran at once
+    for (dg, s), j in zip(reqs, jobs):
+        assert j.status == heipa.OK
+        assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s))
+
+
+def test_pool_rejects_job_larger_than_arena():
+    g = heipa.DeviceGraph(heipa.gen_grid(64, 64), "cuda:0")
+    pool = heipa.Pool(1, 1 << 20)
+    try:
+        j = heipa.Pool.job(g, k=4)
+        pool.submit(j)
+        assert pool.wait(j, 1000.0)
+        assert j.status == heipa.ERR_OOM and j.slot == -1
+    finally:
+        pool.close()

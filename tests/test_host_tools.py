@@ -171,3 +171,36 @@ def test_initial_partitioner_weighted_and_disconnected():
     p = heipa.initial_partition(g, 2, 0.1, 1)
     w = np.bincount(p, weights=g.vwgt, minlength=2)
     assert w.max() <= math.ceil(1.1 * 14 / 2)
+
+
+def test_job_layout_matches_header(tmp_path):
+    """The ctypes mirror of heipa_job has the C layout (offsets from gcc)."""
+    fields = [f for f, _ in heipa.Job._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "heipa.h"\nint main(void) {\n'
+                   + "".join(f'  printf("{f} %zu\\n", offsetof(heipa_job, {f}));\n' for f in fields)
+                   + '  printf("sizeof %zu\\n", sizeof(heipa_job));\n  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(heipa.Job, f).offset, f
+    assert int(out["sizeof"]) == ctypes.sizeof(heipa.Job)
+
+
+def test_job_estimate_grows_with_size():
+    L = heipa.lib()
+    def est(n, m, k, hier=None):
+        j = heipa.Job()
+        j.graph.n, j.graph.m = n, m
+        if hier:
+            j.kind, j.l = heipa.JOB_HM, len(hier)
+            for i, a in enumerate(hier):
+                j.hier[i] = a
+        else:
+            j.kind, j.k = heipa.JOB_PARTITION, k
+        return L.heipa_job_estimate(ctypes.byref(j))
+    assert est(1000, 8000, 8) < est(2000, 8000, 8) < est(2000, 16000, 8)
+    assert est(1 << 20, 12 << 20, 64) > 56 * (12 << 20)
+    assert est(1 << 16, 1 << 20, 8, [4, 8, 6]) > est(1 << 16, 1 << 20, 8)
