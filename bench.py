@@ -46,7 +46,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-all", action="store_true", help="print the per-kernel event profile to stderr")
-    ap.add_argument("--slots", type=int, default=16, help="batched/served: pool workers")
+    ap.add_argument("--slots", type=int, default=8, help="batched/served: pool workers")
+    ap.add_argument("--block-jet-n", type=int, default=1024,
+                    help="batched/served: largest level run by the single-workgroup Jet")
     ap.add_argument("--batch-count", type=int, default=4096, help="batched: number of graphs")
     ap.add_argument("--plain-sample", type=int, default=256, help="batched: graphs timed through the plain path")
     ap.add_argument("--requests", type=int, default=240, help="served: number of requests")
@@ -57,9 +59,6 @@ def main():
               f"across GPUs in this build)", file=sys.stderr)
         sys.exit(2)
     if WORKLOADS[args.workload]["kind"] in ("batched", "served"):
-        # one hardware queue per worker stream, so the workers' kernels overlap
-        # (set before the HIP runtime starts)
-        os.environ["GPU_MAX_HW_QUEUES"] = str(max(4, min(args.slots, 16)))
         return run_pool_workload(args)
 
     import numpy as np
@@ -276,6 +275,7 @@ def run_pool_workload(args):
     plain_rate = sum(edges[i] for i in sample) / plain_s
 
     pool = heipa.Pool(args.slots, arena)
+    pool.set_option(heipa.OPT_BLOCK_JET, args.block_jet_n)
     warm = [heipa.Pool.job(dgs[i], imbalance=0.03, seed=1, **reqs[i][1]) for i in range(min(2 * args.slots, len(reqs)))]
     pool.run(warm)
     del warm
@@ -307,11 +307,22 @@ def run_pool_workload(args):
             pool.submit(j)
         for j in jobs:
             pool.wait(j)
+        # capacity: the same requests all at once (arrivals faster than service)
+        sat = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **spec) for dg, (_, spec, _) in zip(dgs, reqs)]
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        pool.run(sat)
+        sat_s = time.perf_counter() - ts
+        sat_same = sum(1 for a, b, dg in zip(sat, jobs, dgs)
+                       if np.array_equal(heipa.to_numpy_u32(a._part)[:dg.n], heipa.to_numpy_u32(b._part)[:dg.n]))
+        del sat
         lat = np.array([j.t_done - j.t_submit for j in jobs])
         makespan = (max(j.t_done for j in jobs) - min(j.t_submit for j in jobs)) / 1000.0
         value = sum(edges) / makespan
         result.update({"metric": "input edges/sec on 1 MI355X, served form", "value": round(value, 1),
                        "unit": "input edges/s", "ms_per_step": round(makespan * 1000.0, 3),
+                       "capacity": {"value": round(sum(edges) / sat_s, 1), "unit": "input edges/s",
+                                    "seconds": round(sat_s, 3), "identical_to_poisson_run": f"{sat_same}/{len(jobs)}"},
                        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                       "p99": round(float(np.percentile(lat, 99)), 3),
                                       "max": round(float(lat.max()), 3)},
@@ -334,10 +345,18 @@ def run_pool_workload(args):
     result["plain"] = {"value": round(plain_rate, 1), "unit": "input edges/s", "requests": len(sample),
                        "seconds": round(plain_s, 3)}
     result["speedup_vs_plain"] = round(result["value"] / plain_rate, 3)
+    if served:
+        result["capacity_speedup_vs_plain"] = round(result["capacity"]["value"] / plain_rate, 3)
     result["quality"] = {"failed": len(bad), "identical_to_alone": f"{same}/{len(sample)}",
                          "partition_jobs_balanced": balanced,
                          "sum_cut": int(sum(j.cut for j in jobs))}
     result["generation_s"] = round(gen_s, 2)
+    unbalanced = []
+    for i, ((g, spec, kind), j, dg) in enumerate(zip(reqs, jobs, dgs)):
+        if "k" in spec and j.maxw > int(np.ceil(1.03 * g.g_weight / spec["k"])):
+            unbalanced.append({"i": i, "kind": kind, "n": g.n, "k": spec["k"], "maxw": int(j.maxw),
+                               "lmax": int(np.ceil(1.03 * g.g_weight / spec["k"]))})
+    result["quality"]["unbalanced"] = unbalanced[:8]
     print(json.dumps(result), flush=True)
     ctx.close()
 

@@ -370,7 +370,8 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
     return guarded(ctx, [&](Ctx &c) {
         c.sync();
         if (option == HEIPA_OPT_RESOLVE_CAP) c.resolve_cap = value > 0xFFFFFFFFull ? ~0u : (u32) value;
-        else if (option == HEIPA_OPT_BLOCK_JET) c.block_jet = value != 0;
+        else if (option == HEIPA_OPT_BLOCK_JET) c.block_jet_max_n = value > 0xFFFFFFFFull ? ~0u : (u32) value;
+        else if (option == HEIPA_OPT_GRAPHS) c.use_graphs = value != 0;
         else require(false, "unknown option");
     });
 }

@@ -152,7 +152,8 @@ struct Ctx {
     int num_cus = 256;
     size_t max_lds = 65536;  // LDS bytes one workgroup may allocate
     u32 resolve_cap = ~0u;   // HEIPA_OPT_RESOLVE_CAP
-    bool block_jet = true;   // HEIPA_OPT_BLOCK_JET
+    u32 block_jet_max_n = 1024;  // HEIPA_OPT_BLOCK_JET
+    bool use_graphs = true;  // HEIPA_OPT_GRAPHS
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;

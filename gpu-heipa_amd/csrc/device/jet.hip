@@ -787,10 +787,15 @@ __device__ __forceinline__ void classify_body(const JetDev &d, bool strong, u32 
         const u32 i = i0 + threadIdx.x;
         classify_one(d, strong, stamp, i < len, i < len ? region[i] : 0u, mregion, s_cnt);
     }
+    // the extra list in contiguous chunks of ceil(length / workgroups): the
+    // move region has room for exactly that many extra entries
     const u32 lx = (u32) d.ctl->st[ST_CAND];
-    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < lx; i0 += (u64) gridDim.x * blockDim.x) {
-        const u64 i = i0 + threadIdx.x;
-        classify_one(d, strong, stamp, i < lx, i < lx ? d.cand_x[i] : 0u, mregion, s_cnt);
+    const u32 chunk = (lx + gridDim.x - 1) / gridDim.x;
+    const u32 x0 = blockIdx.x * chunk;
+    const u32 x1 = x0 + chunk < lx ? x0 + chunk : lx;
+    for (u32 i0 = x0; i0 < x1; i0 += blockDim.x) {
+        const u32 i = i0 + threadIdx.x;
+        classify_one(d, strong, stamp, i < x1, i < x1 ? d.cand_x[i] : 0u, mregion, s_cnt);
     }
 }
 
@@ -1377,7 +1382,6 @@ constexpr int BM = 1024;
 constexpr u32 BM_PIECE = 2048;
 constexpr u32 BM_RCAP = 2048;
 constexpr size_t BM_CONN_LDS = 32 * 1024;
-constexpr u32 BM_MAX_N = 1024;
 
 __host__ __device__ inline size_t bm_sweep_lds_bytes(u32 k, u32 groups) {
     return conn_group_bytes(k) * groups + 4ull * BM_PIECE + 16 + 4ull * k;
@@ -1923,8 +1927,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         const size_t cgb = conn_group_bytes(k);
         h.bm_groups = (u32) std::max<size_t>(1, std::min<size_t>(BM / meta.G, BM_CONN_LDS / cgb));
         h.bm_lds = std::max(bm_sweep_lds_bytes(k, h.bm_groups), resolve_lds_bytes(BM_RCAP));
-        h.block_mode = n <= BM_MAX_N && meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536) &&
-                       c.block_jet;
+        h.block_mode = n <= c.block_jet_max_n && meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
     }
     d.rcap = std::min<u32>(c.resolve_cap, h.big_resolve ? RESOLVE_CAP : RESOLVE_CAP_SMALL);
     // lane-group geometry: G lanes per vertex; the sweep keeps one LDS
@@ -1972,7 +1975,32 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     launch(c, "jet_init", 0.0, dim3(1), dim3(64), 0, ctl_init_kernel, d, rt, cut, maxw);
 
     // Batches of iterations; the host reads the control block (and the trace
-    // ring) of batch i while batch i + 1 is already queued.
+    // ring) of batch i while batch i + 1 is already queued.  The launches of a
+    // batch never change (all loop state is in device memory), so the
+    // multi-launch form captures one batch of BATCH_MAX iterations as a HIP
+    // graph and replays it: one host call per batch instead of six or more
+    // kernel launches per iteration.
+    struct GraphExec {
+        hipGraphExec_t e = nullptr;
+        ~GraphExec() {
+            if (e) (void) hipGraphExecDestroy(e);
+        }
+    } gx;
+    if (!h.block_mode && c.use_graphs && !c.prof_on) {
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+        try {
+            for (u32 i = 0; i < BATCH_MAX; ++i) h.iteration();
+        } catch (...) {
+            (void) hipStreamEndCapture(c.stream, &graph);
+            if (graph) (void) hipGraphDestroy(graph);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(c.stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&gx.e, graph, nullptr, nullptr, 0);
+        (void) hipGraphDestroy(graph);
+        HIP_CHECK(ie);
+    }
     char *slot[2] = {(char *) c.pinned + 65536, (char *) c.pinned + 65536 + slot_bytes};
     hipEvent_t ev[2] = {c.get_event(), c.get_event()};
     u32 B = 2, consumed = 0;
@@ -1981,6 +2009,8 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     auto submit = [&](int sidx) {
         if (h.block_mode) {
             HEIPA_DISPATCH_G(h.G, h.template launch_block, bm_iters);
+        } else if (gx.e) {
+            HIP_CHECK(hipGraphLaunch(gx.e, c.stream));
         } else {
             for (u32 i = 0; i < B; ++i) h.iteration();
         }

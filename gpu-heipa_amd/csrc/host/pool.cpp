@@ -210,4 +210,14 @@ int heipa_pool_run(heipa_pool *p, heipa_job *jobs, uint32_t count) {
 
 double heipa_pool_clock(const heipa_pool *p) { return p ? p->now() : 0.0; }
 
+int heipa_pool_set_option(heipa_pool *p, int option, uint64_t value) {
+    if (!p) return HEIPA_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(p->m);
+    for (heipa_ctx *c: p->ctxs) {
+        const int rc = heipa_ctx_set_option(c, option, value);
+        if (rc != HEIPA_OK) return rc;
+    }
+    return HEIPA_OK;
+}
+
 } // extern "C"

@@ -22,6 +22,7 @@ HEADER_PATH = os.path.abspath(os.path.join(PKG_ROOT, "..", "include", "heipa.h")
 OK, ERR_OOM, ERR_INVALID, ERR_OVERFLOW, ERR_DEVICE, ERR_IO = 0, 1, 2, 3, 4, 5
 OPT_RESOLVE_CAP = 1
 OPT_BLOCK_JET = 2
+OPT_GRAPHS = 3
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -131,6 +132,7 @@ def lib():
             "heipa_pool_submit": (c_int, [vp, ctypes.POINTER(Job)]),
             "heipa_pool_wait": (c_int, [vp, ctypes.POINTER(Job), c_dbl]),
             "heipa_pool_clock": (c_dbl, [vp]),
+            "heipa_pool_set_option": (c_int, [vp, c_int, c_u64]),
             "heipa_prof_enable": (c_int, [vp, c_int]),
             "heipa_prof_reset": (c_int, [vp]),
             "heipa_prof_filter": (c_int, [vp, ctypes.c_char_p]),
@@ -547,6 +549,11 @@ class Pool:
 
     def clock(self):
         return self.L.heipa_pool_clock(self.h)
+
+    def set_option(self, option, value):
+        rc = self.L.heipa_pool_set_option(self.h, int(option), int(value))
+        if rc != OK:
+            raise HeipaError("heipa_pool_set_option", rc)
 
     def close(self):
         if self.h:

@@ -190,6 +190,8 @@ int heipa_pool_submit(heipa_pool *pool, heipa_job *job);
 /* waits until the job is done or timeout_ms passes (negative: no limit); 1 done, 0 not yet */
 int heipa_pool_wait(heipa_pool *pool, heipa_job *job, double timeout_ms);
 double heipa_pool_clock(const heipa_pool *pool);
+/* heipa_ctx_set_option on every worker's context (call while no job runs) */
+int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
 
 /* ---- context options.  No counterpart in the reference (its sizes are compile-time).
  * HEIPA_OPT_RESOLVE_CAP: most rebalance resolve entries one workgroup sorts in LDS before an
@@ -197,9 +199,12 @@ double heipa_pool_clock(const heipa_pool *pool);
  * allows; 0 forces the device-wide path).  Results do not depend on it; tests use it to cover
  * both paths. */
 #define HEIPA_OPT_RESOLVE_CAP 1
-/* HEIPA_OPT_BLOCK_JET: 1 (default) lets Jet run levels of at most 1024 vertices without hubs in
- * one workgroup; 0 forces the multi-launch form everywhere.  Results do not depend on it. */
+/* HEIPA_OPT_BLOCK_JET: the largest level (vertices) whose Jet runs in one workgroup when it has no
+ * hubs (default 1024; 0 forces the multi-launch form everywhere).  Results do not depend on it. */
 #define HEIPA_OPT_BLOCK_JET 2
+/* HEIPA_OPT_GRAPHS: 1 (default) replays each level's batch of Jet iterations as a captured HIP graph;
+ * 0 launches every kernel.  Results do not depend on it. */
+#define HEIPA_OPT_GRAPHS 3
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */

@@ -62,3 +62,34 @@ def random_partition(n, k, seed, skew=0.0):
 def block_weights(g, part, k):
     vw = np.ones(g.n, np.int64) if g.vwgt is None else g.vwgt.astype(np.int64)
     return np.bincount(part, weights=vw, minlength=k).astype(np.int64)
+
+
+def csr_from_arrays(n, us, vs, ws=None):
+    """Symmetric CSR (sorted rows, no self loops) from undirected edge arrays;
+    a pair given more than once keeps its first weight, in both directions."""
+    us, vs = np.asarray(us, np.int64), np.asarray(vs, np.int64)
+    keep = us != vs
+    us, vs = us[keep], vs[keep]
+    lo, hi = np.minimum(us, vs), np.maximum(us, vs)
+    _, first = np.unique(lo * n + hi, return_index=True)
+    lo, hi = lo[first], hi[first]
+    w = None if ws is None else np.asarray(ws, np.int64)[keep][first]
+    a, b = np.concatenate([lo, hi]), np.concatenate([hi, lo])
+    order = np.argsort(a * n + b, kind="stable")
+    a, b = a[order], b[order]
+    xadj = np.zeros(n + 1, np.int64)
+    np.add.at(xadj, a + 1, 1)
+    xadj = np.cumsum(xadj).astype(np.uint32)
+    ew = None if w is None else np.concatenate([w, w])[order].astype(np.int32)
+    return heipa.HostGraph(xadj, b.astype(np.uint32), ew, None)
+
+
+def many_hubs(n=60000, hubs=100, hub_degree=3000, seed=5):
+    """A ring of n vertices plus `hubs` vertices of degree about hub_degree to
+    random vertices: many workgroup-per-hub vertices at once."""
+    rng = np.random.default_rng(seed)
+    ring = np.arange(n)
+    us = [ring, np.repeat(np.arange(hubs), hub_degree)]
+    vs = [(ring + 1) % n, rng.integers(0, n, hubs * hub_degree)]
+    u, v = np.concatenate(us), np.concatenate(vs)
+    return csr_from_arrays(n, u, v, rng.integers(1, 10, len(u)))

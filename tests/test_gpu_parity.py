@@ -7,7 +7,7 @@ import pytest
 
 import heipa
 import heipa_oracle as orc
-from graphs import block_weights, random_partition, star_plus_grid
+from graphs import block_weights, many_hubs, random_partition, star_plus_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,7 @@ def _graphs():
         "rmat12w": heipa.gen_rmat(12, 0, 16.0, 0.57, 0.19, 0.19, 2, 100, True),
         "rmat11u": heipa.gen_rmat(11, 0, 8.0, 0.57, 0.19, 0.19, 3, 0, False),
         "star": star_plus_grid(),
+        "hubs100": many_hubs(),
     }
 
 
@@ -119,19 +120,24 @@ def test_jet(ctx, graph, ultra, level, skew, k):
     assert orc.edge_cut(g, gp) == gcut
 
 
+@pytest.mark.parametrize("graphs", [1, 0])
 @pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.0, 8), (True, 0, 0.0, 8), (False, 0, 0.6, 8),
                                                 (True, 2, 0.9, 8), (False, 0, 0.3, 3), (False, 1, 0.7, 7)])
-def test_jet_multi_launch(ctx, graph, ultra, level, skew, k):
+def test_jet_multi_launch(ctx, graph, graphs, ultra, level, skew, k):
     """Small graphs run Jet in one workgroup by default; this forces the
-    multi-launch form on them."""
+    multi-launch form on them, with batches replayed from a captured HIP graph
+    (every replay is checked through the per-iteration trace) or launched
+    kernel by kernel."""
     ctx.set_option(heipa.OPT_BLOCK_JET, 0)
+    ctx.set_option(heipa.OPT_GRAPHS, graphs)
     try:
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
-        ctx.set_option(heipa.OPT_BLOCK_JET, 1)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
+        ctx.set_option(heipa.OPT_GRAPHS, 1)
 
 
-@pytest.mark.parametrize("block", [1, 0])
+@pytest.mark.parametrize("block", [1 << 20, 0])
 @pytest.mark.parametrize("cap", [0, 16])
 @pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 1, 0.7, 7)])
 def test_jet_global_resolve(ctx, graph, cap, block, ultra, level, skew, k):
@@ -144,7 +150,7 @@ def test_jet_global_resolve(ctx, graph, cap, block, ultra, level, skew, k):
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
         ctx.set_option(heipa.OPT_RESOLVE_CAP, 1 << 32)
-        ctx.set_option(heipa.OPT_BLOCK_JET, 1)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
 
 
 @pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False), (3, False), (7, True)])
