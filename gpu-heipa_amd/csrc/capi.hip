@@ -372,6 +372,7 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         if (option == HEIPA_OPT_RESOLVE_CAP) c.resolve_cap = value > 0xFFFFFFFFull ? ~0u : (u32) value;
         else if (option == HEIPA_OPT_BLOCK_JET) c.block_jet_max_n = value > 0xFFFFFFFFull ? ~0u : (u32) value;
         else if (option == HEIPA_OPT_GRAPHS) c.use_graphs = value != 0;
+        else if (option == HEIPA_OPT_PROF_SAMPLE) c.prof_sample = value < 1 ? 1u : (u32) std::min<uint64_t>(value, 1u << 30);
         else require(false, "unknown option");
     });
 }
@@ -410,7 +411,8 @@ int heipa_prof_get(heipa_ctx *ctx, uint32_t idx, char *name, uint32_t name_cap, 
         }
         const KernelStat &s = c.prof_stats[idx];
         if (launches) *launches = s.launches;
-        if (total_ms) *total_ms = s.ms;
+        // the timed launches' mean, extrapolated to every launch
+        if (total_ms) *total_ms = s.timed ? s.ms * (double) s.launches / (double) s.timed : 0.0;
         if (alg_bytes) *alg_bytes = s.bytes;
     });
 }

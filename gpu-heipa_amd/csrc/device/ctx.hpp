@@ -142,8 +142,9 @@ struct BackScope {
 };
 
 struct KernelStat {
-    u64 launches = 0;
-    double ms = 0.0;
+    u64 launches = 0;  // all launches
+    u64 timed = 0;     // launches bracketed by events (every prof_sample-th)
+    double ms = 0.0;   // event time of the timed launches
     double bytes = 0.0;
 };
 
@@ -164,13 +165,13 @@ struct Ctx {
     // profiling
     bool prof_on = false;
     std::string prof_filter;  // record only kernels whose name starts with this (empty: all)
+    u32 prof_sample = 1;      // bracket about one launch in prof_sample of a kernel with events
     std::vector<std::string> prof_names;
     std::map<std::string, size_t> prof_index;
     std::vector<KernelStat> prof_stats;
     struct Pending {
         size_t idx;
         hipEvent_t a, b;
-        double bytes;
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
@@ -200,9 +201,8 @@ struct Ctx {
             float ms = 0.f;
             HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
             auto &s = prof_stats[p.idx];
-            s.launches++;
+            s.timed++;
             s.ms += ms;
-            s.bytes += p.bytes;
             event_pool.push_back(p.a);
             event_pool.push_back(p.b);
         }
@@ -231,7 +231,19 @@ inline void launch(Ctx &c, const char *name, double bytes, dim3 grid, dim3 block
                    void (*kernel)(KArgs...), Args... args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
     hipEvent_t ea = nullptr, eb = nullptr;
-    const bool timed = c.prof_on && (c.prof_filter.empty() || std::strncmp(name, c.prof_filter.c_str(), c.prof_filter.size()) == 0);
+    bool timed = false;
+    size_t slot = 0;
+    if (c.prof_on && (c.prof_filter.empty() || std::strncmp(name, c.prof_filter.c_str(), c.prof_filter.size()) == 0)) {
+        slot = c.prof_slot(name);
+        KernelStat &st = c.prof_stats[slot];
+        st.launches++;
+        st.bytes += bytes;
+        // about one launch in prof_sample, chosen by a hash of the launch count so
+        // that periodic launch patterns (LP / rebalance alternation) do not bias it
+        u64 z = st.launches * 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 31)) * 0xbf58476d1ce4e5b9ull;
+        timed = c.prof_sample <= 1 || (z ^ (z >> 29)) % c.prof_sample == 0;
+    }
     if (timed) {
         ea = c.get_event();
         eb = c.get_event();
@@ -241,7 +253,7 @@ inline void launch(Ctx &c, const char *name, double bytes, dim3 grid, dim3 block
     HIP_CHECK(hipGetLastError());
     if (timed) {
         HIP_CHECK(hipEventRecord(eb, c.stream));
-        c.pending.push_back({c.prof_slot(name), ea, eb, bytes});
+        c.pending.push_back({slot, ea, eb});
         if (c.pending.size() > 4096) c.sync();
     }
 }

@@ -23,6 +23,7 @@ OK, ERR_OOM, ERR_INVALID, ERR_OVERFLOW, ERR_DEVICE, ERR_IO = 0, 1, 2, 3, 4, 5
 OPT_RESOLVE_CAP = 1
 OPT_BLOCK_JET = 2
 OPT_GRAPHS = 3
+OPT_PROF_SAMPLE = 4
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -205,6 +205,10 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
 /* HEIPA_OPT_GRAPHS: 1 (default) replays each level's batch of Jet iterations as a captured HIP graph;
  * 0 launches every kernel.  Results do not depend on it. */
 #define HEIPA_OPT_GRAPHS 3
+/* HEIPA_OPT_PROF_SAMPLE: while profiling, bracket about one launch in `value` of a kernel with HIP
+ * events, chosen by a hash of the launch count (default 1 = every launch); heipa_prof_get extrapolates
+ * the timed launches' mean to all launches. */
+#define HEIPA_OPT_PROF_SAMPLE 4
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */
@@ -213,7 +217,8 @@ int heipa_prof_enable(heipa_ctx *ctx, int on);
 int heipa_prof_filter(heipa_ctx *ctx, const char *prefix);
 int heipa_prof_reset(heipa_ctx *ctx);
 uint32_t heipa_prof_count(heipa_ctx *ctx);
-/* algorithmic bytes = the compulsory HBM bytes of each launch, summed */
+/* algorithmic bytes = the compulsory HBM bytes of each launch, summed; total_ms = the mean of the
+ * event-timed launches times the number of launches (see HEIPA_OPT_PROF_SAMPLE) */
 int heipa_prof_get(heipa_ctx *ctx, uint32_t idx, char *name, uint32_t name_cap, uint64_t *launches,
                    double *total_ms, double *alg_bytes);
 
