@@ -685,6 +685,107 @@ struct Jet {
         return moves;
     }
 
+    // ---- mapping objective (promap_jet_label_propagation.h, "promap" below)
+    const s32 *D = nullptr;  // k x k distances, row-major
+    s64 dist(u32 a, u32 b) const { return (s64) D[(size_t) a * k + b]; }
+
+    // sum over the blocks c adjacent to u of conn(u, c) * D(x, c): gain_delta
+    // (promap:180-204) is J_x(old) - J_x(new) over the canonical connection set
+    s64 conn_cost(u32 x) const {
+        s64 s = 0;
+        for (u32 c: conn.touched) s += conn.acc[c] * dist(x, c);
+        return s;
+    }
+
+    // ProMap_jet_lp (promap:206-330); returns the move list
+    std::vector<u32> lp_map(u32 &n_cand, u64 &gain_hash) {
+        for (u32 u = 0; u < g.n; ++u) {  // best_block :219-260
+            if (dest_cache[u] != NULL_PART) {
+                dest_part[u] = dest_cache[u];
+                continue;
+            }
+            const u32 u_id = part[u];
+            conn.compute(g, part, u);
+            const s64 own_cost = conn_cost(u_id);
+            u32 best_id = NO_MOVE;
+            s64 best_delta = -1;  // :232: a move needs delta >= 0
+            for (u32 b: conn.touched) {  // D4
+                if (b == u_id) continue;
+                const s64 delta = own_cost - conn_cost(b);
+                if (delta > best_delta || (delta == best_delta && best_id != NO_MOVE && b < best_id)) {
+                    best_delta = delta;
+                    best_id = b;
+                }
+            }
+            conn.reset();
+            gain_cache[u] = best_id != NO_MOVE ? best_delta : 0;
+            dest_cache[u] = best_id;
+            dest_part[u] = best_id;
+        }
+        gain_hash = 0;
+        std::vector<u32> cand;
+        for (u32 u = 0; u < g.n; ++u) {  // :262-269
+            if (dest_part[u] != NO_MOVE && lock[u] == 0) {
+                cand.push_back(u);
+                gain1[u] = gain_cache[u];
+                gain_hash += mix64(mix64(u) + (u64) gain1[u]);
+            } else {
+                gain1[u] = GAIN_MIN;
+                lock[u] = 0;
+            }
+        }
+        n_cand = (u32) cand.size();
+        std::vector<u32> moves;
+        for (u32 u: cand) {  // afterburner :284-322
+            const s64 ug = gain1[u];
+            const u32 o = part[u], nn = dest_part[u];
+            s64 change = 0;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                const s64 vg = gain1[v];
+                if (!(vg > ug || (vg == ug && v < u))) continue;
+                const u32 vn = dest_part[v], vo = part[v];
+                const s64 before_u = dist(o, vo) - dist(nn, vo);
+                const s64 after_u = dist(o, vn) - dist(nn, vn);
+                change += g.w(j) * (after_u - before_u);
+            }
+            if (ug + change >= 0) {
+                lock[u] = 1;
+                moves.push_back(u);
+            }
+        }
+        return moves;
+    }
+
+    // ProMap_perform_moves (promap:1104-1216): J change over the moved
+    // vertices' edges, doubled where the other end did not change block
+    void apply_map(const std::vector<u32> &moves) {
+        std::vector<u8> changed(g.n, 0);
+        for (u32 u: moves) changed[u] = dest_part[u] != part[u];
+        s64 delta = 0;
+        for (u32 u: moves) {
+            const u32 ou = part[u], nu = dest_part[u];
+            if (ou == nu) continue;  // :1180
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                const u32 ov = part[v], nv = changed[v] ? dest_part[v] : part[v];
+                const s64 x = g.w(j) * (dist(nu, nv) - dist(ou, ov));
+                delta += changed[v] ? x : 2 * x;
+            }
+        }
+        cut += delta;
+        for (u32 u: moves) {
+            const u32 ou = part[u], nu = dest_part[u];
+            bw[ou] -= g.v(u);
+            bw[nu] += g.v(u);
+            dest_cache[u] = NULL_PART;  // :1133, block_conn.h update_small/update_large
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) dest_cache[g.adj[j]] = NULL_PART;
+        }
+        for (u32 u: moves) part[u] = dest_part[u];
+        maxw = 0;
+        for (u32 b = 0; b < k; ++b) maxw = std::max(maxw, bw[b]);
+    }
+
     // perform_moves (jet:753-924) with the exact 64-bit cut change
     void apply(const std::vector<u32> &moves) {
         std::vector<u8> moved(g.n, 0);
@@ -796,6 +897,80 @@ void jet_refine(const G &g, std::vector<u32> &part, u32 k, s64 lmax, bool ultra,
     }
     if (trace_len) *trace_len = tl;
     cut = best_cut;
+    maxw = best_maxw;
+}
+
+// ProMap_jet_refine (promap:1219-1311) followed literally: one filter ratio
+// (which ProMap_jet_lp does not read), the balance counter incremented before
+// each rebalance iteration (one weak, then strong), iterations without moves
+// counted and skipped, and a strict `<` on the max block weight in the second
+// best-partition test.  `J` is the working comm cost; part is in/out.
+void map_refine(const G &g, std::vector<u32> &part, const s32 *D, u32 k, s64 lmax, s64 &J, s64 &maxw,
+                orc_jet_trace *trace, u32 trace_cap, u32 *trace_len) {
+    Jet X(g, k, lmax);
+    X.D = D;
+    X.part = part;
+    X.bw.assign(k, 0);
+    for (u32 u = 0; u < g.n; ++u) X.bw[part[u]] += g.v(u);
+    X.dest_part.assign(g.n, 0);
+    X.dest_cache.assign(g.n, NULL_PART);
+    X.gain_cache.assign(g.n, 0);
+    X.gain1.assign(g.n, 0);
+    X.lock.assign(g.n, 0);
+    X.cut = J;
+    X.maxw = maxw;
+    s64 best_J = J, best_maxw = maxw;
+    u32 tl = 0, bal = 0, it = 0;
+    while (it < N_MAX_ITERATIONS) {
+        it += 1;
+        std::vector<u32> moves;
+        u32 mode, n_cand = 0;
+        u64 gh = 0;
+        if (X.maxw <= lmax) {
+            mode = 0;
+            moves = X.lp_map(n_cand, gh);
+            bal = 0;
+        } else {
+            bal++;
+            if (bal < N_MAX_WEAK_ITERATIONS) {
+                mode = 1;
+                moves = X.weak(n_cand);
+            } else {
+                mode = 2;
+                moves = X.strong(n_cand);
+            }
+        }
+        if (moves.empty()) continue;
+        const u64 mh = moves_hash(moves, X.dest_part);
+        X.apply_map(moves);
+        if (trace && tl < trace_cap) {
+            orc_jet_trace &t = trace[tl];
+            t.ratio_idx = 0;
+            t.iteration = it;
+            t.mode = mode;
+            t.n_cand = n_cand;
+            t.n_moves = (u32) moves.size();
+            t.pad = 0;
+            t.cut = X.cut;
+            t.maxw = X.maxw;
+            t.moves_hash = mh;
+            t.gain_hash = gh;
+        }
+        ++tl;
+        if (best_maxw > lmax && X.maxw < best_maxw) {
+            part = X.part;
+            best_J = X.cut;
+            best_maxw = X.maxw;
+            it = 0;
+        } else if (X.cut < best_J && (X.maxw <= lmax || X.maxw < best_maxw)) {
+            if ((double) X.cut < PHI * (double) best_J) it = 0;
+            part = X.part;
+            best_J = X.cut;
+            best_maxw = X.maxw;
+        }
+    }
+    if (trace_len) *trace_len = tl;
+    J = best_J;
     maxw = best_maxw;
 }
 
@@ -924,6 +1099,143 @@ bool hm_rec(const G &g, const std::vector<u32> &n_to_o, const std::vector<u32> &
     return true;
 }
 
+// recursive_partition (global_multisection.h:139-215): the shared host
+// partitioner (standing in for kway_partition) splits g into hier[level]
+// blocks under the adaptive imbalance; build_subgraphs (:50-137) keeps each
+// block's vertices in id order with their weights and the block-internal
+// edges in adjacency order.  An empty block has no vertices to place and is
+// skipped (the reference recurses into it with zero weight).
+void gms_rec(const G &g, const std::vector<u32> &n_to_o, const std::vector<u32> &hier, u32 level, double eps,
+             s64 gW, u32 gk, u64 seed, const std::vector<u32> &index_vec, const std::vector<u32> &k_rem,
+             std::vector<u32> &ident, std::vector<u32> &global) {
+    const u32 l = (u32) hier.size();
+    const u32 k = hier[level];
+    const double imb = adaptive_imbalance(eps, gW, gk, g.gw, k_rem[l - 1 - ident.size()], l - ident.size());
+    std::vector<u32> tmp(g.n, 0);
+    heipa::initial_partition(g.n, g.xadj.data(), g.adj.data(), g.uew ? nullptr : g.ew.data(),
+                             g.uvw ? nullptr : g.vw.data(), k, imb, seed, tmp.data());
+    if (ident.size() == l - 1) {  // :169-177
+        u32 offset = 0;
+        for (u32 i = 0; i + 1 < l; ++i) offset += ident[i] * index_vec[index_vec.size() - 1 - i];
+        for (u32 u = 0; u < g.n; ++u) global[n_to_o[u]] = offset + tmp[u];
+        return;
+    }
+    for (u32 id = 0; id < k; ++id) {
+        G c;
+        std::vector<u32> loc(g.n, SENT), c_n_to_o;
+        for (u32 u = 0; u < g.n; ++u) {
+            if (tmp[u] != id) continue;
+            loc[u] = c.n++;
+            c_n_to_o.push_back(n_to_o[u]);
+        }
+        if (c.n == 0) continue;
+        c.uew = false;
+        c.uvw = false;
+        c.xadj.assign(c.n + 1, 0);
+        s64 sw = 0;
+        for (u32 u = 0; u < g.n; ++u) {
+            if (tmp[u] != id) continue;
+            c.vw.push_back((s32) g.v(u));
+            sw += g.v(u);
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j];
+                if (tmp[v] != id) continue;
+                c.adj.push_back(loc[v]);
+                c.ew.push_back((s32) g.w(j));
+            }
+            c.xadj[loc[u] + 1] = (u32) c.adj.size();
+        }
+        c.m = (u32) c.adj.size();
+        c.gw = sw;
+        ident.push_back(id);
+        gms_rec(c, c_n_to_o, hier, level - 1, eps, gW, gk, seed, index_vec, k_rem, ident, global);
+        ident.pop_back();
+    }
+}
+
+// global_multisection_host (global_multisection.h:217-259)
+void global_multisection(const G &g, const std::vector<u32> &hier, double eps, u64 seed, std::vector<u32> &part) {
+    const u32 l = (u32) hier.size();
+    u32 gk = 1;
+    for (u32 x: hier) gk *= x;
+    std::vector<u32> index_vec = {1};  // :229-230
+    for (u32 i = 0; i + 1 < l; ++i) index_vec.push_back(index_vec[i] * hier[i]);
+    std::vector<u32> k_rem(l);  // :232-237
+    u32 p = 1;
+    for (u32 i = 0; i < l; ++i) {
+        k_rem[i] = p * hier[i];
+        p *= hier[i];
+    }
+    std::vector<u32> n_to_o(g.n), ident;
+    std::iota(n_to_o.begin(), n_to_o.end(), 0u);
+    part.assign(g.n, 0);
+    gms_rec(g, n_to_o, hier, l - 1, eps, g.gw, gk, seed, index_vec, k_rem, ident, part);
+}
+
+// ProMapSolver::internal_solve (promap_solver.h:216-246, :248-330): the
+// Solver's coarsening, global multisection of the coarsest graph, then
+// projection and map_refine level by level with J tracked incrementally.
+bool im_solve(const G &g0, const std::vector<u32> &hier, const std::vector<s32> &dist, double imbalance, u64 seed,
+              SolveOut &out, std::vector<orc_level_trace> *levels) {
+    u32 k = 1;
+    for (u32 x: hier) k *= x;
+    std::vector<s32> D((size_t) k * k);
+    orc_distance_matrix(hier.data(), dist.data(), (u32) hier.size(), D.data());
+    const s64 lmax = (s64) std::ceil((1.0 + imbalance) * ((double) g0.gw / (double) k));  // promap_solver.h:299
+    std::vector<G> graphs;
+    std::vector<std::vector<u32>> maps;
+    graphs.push_back(g0);
+    const u32 max_n = 8 * k;  // promap_solver.h:220-221
+    while (graphs.back().n > max_n) {
+        std::vector<u32> mp;
+        u32 nc = match(graphs.back(), 12345u, mp, nullptr);
+        G c;
+        if (!contract(graphs.back(), mp, nc, c)) return false;
+        maps.push_back(std::move(mp));
+        graphs.push_back(std::move(c));
+    }
+    const G &cg = graphs.back();
+    std::vector<u32> part;
+    global_multisection(cg, hier, imbalance, seed, part);  // :336-347
+    s64 J = comm_cost(cg, part.data(), D.data(), k);
+    s64 maxw = 0;
+    {
+        std::vector<s64> bw(k, 0);
+        for (u32 u = 0; u < cg.n; ++u) bw[part[u]] += cg.v(u);
+        for (u32 b = 0; b < k; ++b) maxw = std::max(maxw, bw[b]);
+    }
+    if (levels) {
+        levels->assign(graphs.size(), orc_level_trace{});
+        for (size_t l = 0; l < graphs.size(); ++l) {
+            auto &L = (*levels)[l];
+            L.n = graphs[l].n;
+            L.m = graphs[l].m;
+            L.nc = l + 1 < graphs.size() ? graphs[l + 1].n : 0;
+            if (l < maps.size()) {
+                u64 h = 0;
+                for (u32 u = 0; u < graphs[l].n; ++u) h += mix64(((u64) u << 32) | maps[l][u]);
+                L.mapping_hash = h;
+            }
+        }
+        levels->back().cut = J;
+        levels->back().maxw = maxw;
+    }
+    for (size_t l = maps.size(); l-- > 0;) {
+        std::vector<u32> fine(graphs[l].n);
+        for (u32 u = 0; u < graphs[l].n; ++u) fine[u] = part[maps[l][u]];
+        part.swap(fine);
+        map_refine(graphs[l], part, D.data(), k, lmax, J, maxw, nullptr, 0, nullptr);
+        if (levels) {
+            (*levels)[l].cut = J;
+            (*levels)[l].maxw = maxw;
+        }
+    }
+    out.part = std::move(part);
+    out.cut = J;
+    out.maxw = maxw;
+    return true;
+}
+
 } // namespace
 
 extern "C" {
@@ -1007,6 +1319,48 @@ int orc_partition(const orc_graph *g, uint32_t k, double imbalance, uint64_t see
     if (!solve(gg, k, imbalance, seed, ultra != 0, so, levels ? &lv : nullptr)) return 3;
     std::memcpy(part, so.part.data(), sizeof(u32) * gg.n);
     if (cut) *cut = so.cut;
+    if (maxw) *maxw = so.maxw;
+    if (levels) {
+        for (u32 i = 0; i < lv.size() && i < level_cap; ++i) levels[i] = lv[i];
+        if (n_levels) *n_levels = (u32) lv.size();
+    }
+    return 0;
+}
+
+int orc_map_refine(const orc_graph *g, uint32_t *part, const int32_t *D, uint32_t k, int64_t lmax, int64_t *J,
+                   int64_t *maxw, orc_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len) {
+    G gg = from_view(g);
+    std::vector<u32> p(part, part + gg.n);
+    s64 j = *J, mw = *maxw;
+    map_refine(gg, p, D, k, lmax, j, mw, trace, trace_cap, trace_len);
+    std::memcpy(part, p.data(), sizeof(u32) * gg.n);
+    *J = j;
+    *maxw = mw;
+    return 0;
+}
+
+int orc_global_multisection(const orc_graph *g, const uint32_t *hier, uint32_t l, double imbalance, uint64_t seed,
+                            uint32_t *part) {
+    if (l == 0) return 2;
+    G gg = from_view(g);
+    std::vector<u32> p;
+    global_multisection(gg, std::vector<u32>(hier, hier + l), imbalance, seed, p);
+    std::memcpy(part, p.data(), sizeof(u32) * gg.n);
+    return 0;
+}
+
+int orc_im_partition(const orc_graph *g, const uint32_t *hier, const int32_t *dist, uint32_t l, double imbalance,
+                     uint64_t seed, uint32_t *part, int64_t *J, int64_t *maxw, orc_level_trace *levels,
+                     uint32_t level_cap, uint32_t *n_levels) {
+    if (l == 0) return 2;
+    G gg = from_view(g);
+    SolveOut so;
+    std::vector<orc_level_trace> lv;
+    if (!im_solve(gg, std::vector<u32>(hier, hier + l), std::vector<s32>(dist, dist + l), imbalance, seed, so,
+                  levels ? &lv : nullptr))
+        return 3;
+    std::memcpy(part, so.part.data(), sizeof(u32) * gg.n);
+    if (J) *J = so.cut;
     if (maxw) *maxw = so.maxw;
     if (levels) {
         for (u32 i = 0; i < lv.size() && i < level_cap; ++i) levels[i] = lv[i];

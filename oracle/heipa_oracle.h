@@ -95,6 +95,17 @@ int orc_partition(const orc_graph *g, uint32_t k, double imbalance, uint64_t see
 /* hierarchical_multisection.h:237-307 */
 int orc_hm_partition(const orc_graph *g, const uint32_t *hier, uint32_t l, double imbalance, uint64_t seed,
                      int ultra, uint32_t *part);
+/* ProMap_jet_refine (promap_jet_label_propagation.h:1219-1311); *J is the comm cost in/out,
+ * trace records carry J in their cut field */
+int orc_map_refine(const orc_graph *g, uint32_t *part, const int32_t *D, uint32_t k, int64_t lmax, int64_t *J,
+                   int64_t *maxw, orc_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len);
+/* global_multisection_host (global_multisection.h:217-259) over the shared host partitioner */
+int orc_global_multisection(const orc_graph *g, const uint32_t *hier, uint32_t l, double imbalance, uint64_t seed,
+                            uint32_t *part);
+/* ProMapSolver::solve, --config IM (promap_solver.h:146-246); level trace cut fields hold J */
+int orc_im_partition(const orc_graph *g, const uint32_t *hier, const int32_t *dist, uint32_t l, double imbalance,
+                     uint64_t seed, uint32_t *part, int64_t *J, int64_t *maxw, orc_level_trace *levels,
+                     uint32_t level_cap, uint32_t *n_levels);
 /* global_multisection.h:39-48 */
 double orc_adaptive_imbalance(double global_imbalance, int64_t global_g_weight, uint32_t global_k,
                               int64_t local_g_weight, uint32_t local_k_rem, uint64_t depth);

@@ -75,6 +75,11 @@ def lib():
                                       ctypes.POINTER(OrcLevelTrace), c_u32, u32p]),
             "orc_hm_partition": (c_int, [G, u32p, c_u32, c_dbl, c_u64, c_int, u32p]),
             "orc_adaptive_imbalance": (c_dbl, [c_dbl, ctypes.c_int64, c_u32, ctypes.c_int64, c_u32, c_u64]),
+            "orc_map_refine": (c_int, [G, u32p, s32p, c_u32, ctypes.c_int64, s64p, s64p,
+                                       ctypes.POINTER(OrcJetTrace), c_u32, u32p]),
+            "orc_global_multisection": (c_int, [G, u32p, c_u32, c_dbl, c_u64, u32p]),
+            "orc_im_partition": (c_int, [G, u32p, s32p, c_u32, c_dbl, c_u64, u32p, s64p, s64p,
+                                         ctypes.POINTER(OrcLevelTrace), c_u32, u32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -228,3 +233,48 @@ def hm_partition(g, hier, imbalance=0.03, seed=1, ultra=False):
 
 def adaptive_imbalance(eps, gW, gk, lW, krem, depth):
     return lib().orc_adaptive_imbalance(eps, gW, gk, lW, krem, depth)
+
+
+def map_refine(g, part, D, lmax, J, maxw, trace_cap=0):
+    """ProMap_jet_refine: returns (part, J, maxw, trace); trace cut fields hold J."""
+    G = _G(g)
+    p = np.ascontiguousarray(part, np.uint32).copy()
+    D = np.ascontiguousarray(D, np.int32)
+    k = int(round(np.sqrt(D.size)))
+    j, mw = ctypes.c_int64(J), ctypes.c_int64(maxw)
+    tr = (OrcJetTrace * trace_cap)() if trace_cap else None
+    tl = ctypes.c_uint32()
+    lib().orc_map_refine(G.ref, _p(p, ctypes.c_uint32), _p(D.reshape(-1), ctypes.c_int32), k, lmax, ctypes.byref(j),
+                         ctypes.byref(mw), tr, trace_cap, ctypes.byref(tl))
+    trace = [tr[i] for i in range(min(tl.value, trace_cap))] if trace_cap else None
+    return p, j.value, mw.value, trace
+
+
+def global_multisection(g, hier, imbalance=0.03, seed=1):
+    G = _G(g)
+    n = len(G.xadj) - 1
+    p = np.zeros(max(n, 1), np.uint32)
+    h = np.ascontiguousarray(hier, np.uint32)
+    rc = lib().orc_global_multisection(G.ref, _p(h, ctypes.c_uint32), len(h), imbalance, seed,
+                                       _p(p, ctypes.c_uint32))
+    if rc != 0:
+        raise RuntimeError(f"orc_global_multisection failed: {rc}")
+    return p[:n]
+
+
+def im_partition(g, hier, dist, imbalance=0.03, seed=1, level_cap=64):
+    """ProMapSolver::solve (--config IM): returns (part, J, maxw, levels)."""
+    G = _G(g)
+    n = len(G.xadj) - 1
+    p = np.zeros(max(n, 1), np.uint32)
+    h = np.ascontiguousarray(hier, np.uint32)
+    d = np.ascontiguousarray(dist, np.int32)
+    j, mw = ctypes.c_int64(), ctypes.c_int64()
+    lv = (OrcLevelTrace * level_cap)()
+    nl = ctypes.c_uint32()
+    rc = lib().orc_im_partition(G.ref, _p(h, ctypes.c_uint32), _p(d, ctypes.c_int32), len(h), imbalance, seed,
+                                _p(p, ctypes.c_uint32), ctypes.byref(j), ctypes.byref(mw), lv, level_cap,
+                                ctypes.byref(nl))
+    if rc != 0:
+        raise RuntimeError(f"orc_im_partition failed: {rc}")
+    return p[:n], j.value, mw.value, [lv[i] for i in range(min(nl.value, level_cap))]

@@ -1,0 +1,217 @@
+"""CPU pins of the oracle's IM process-mapping path (ProMap_jet_refine, global
+multisection, ProMapSolver::solve).
+
+The reference ships no tests or fixtures for this path, so the oracle is
+pinned by what the mathematics fixes:
+  * the LP gain of a move (promap_jet_label_propagation.h:180-204) is half the
+    drop of J = sum over directed edges w * D(block(u), block(v)) when the
+    vertex moves alone, and the afterburner's change (:284-322) is the change
+    of that gain once the neighbours ordered before it have moved: both are
+    recomputed here from whole-graph J evaluations, never from the formula;
+  * the incrementally tracked J equals a recount (the reference's own
+    after-refinement assert, promap_solver.h via assert_state_after_partition);
+  * with one hierarchy level and distance 1, J is twice the edge cut;
+  * global multisection is restated a second time here in Python over the
+    shared host partitioner and must agree entry for entry.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import heipa
+import heipa_oracle as orc
+from graphs import block_weights, random_partition
+
+GAIN_MIN = -(1 << 63)
+U64 = (1 << 64) - 1
+
+
+def _J(g, part, D):
+    """Brute-force comm cost: every directed CSR entry."""
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj.astype(np.int64)))
+    w = np.ones(len(g.adjncy), np.int64) if g.adjwgt is None else g.adjwgt.astype(np.int64)
+    return int((w * D[part[src], part[g.adjncy.astype(np.int64)]].astype(np.int64)).sum())
+
+
+def _moved(part, moves):
+    p = part.copy()
+    for u, b in moves.items():
+        p[u] = b
+    return p
+
+
+def _first_lp_iteration(g, part, D):
+    """One ProMap LP iteration from scratch (no cache, no locks) by brute force."""
+    J0 = _J(g, part, D)
+    gain, dest = {}, {}
+    for u in range(g.n):
+        blocks = sorted({int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]} - {int(part[u])})
+        best, best_b = -1, None
+        for b in blocks:
+            d2 = J0 - _J(g, _moved(part, {u: b}), D)
+            assert d2 % 2 == 0
+            if d2 // 2 > best:
+                best, best_b = d2 // 2, b
+        if best_b is not None:
+            gain[u], dest[u] = best, best_b
+    moves = []
+    for u in sorted(gain):
+        first = {int(v): dest[int(v)] for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]
+                 if int(v) in gain and (gain[int(v)] > gain[u] or (gain[int(v)] == gain[u] and v < u))}
+        p1 = _moved(part, first)
+        after = (_J(g, p1, D) - _J(g, _moved(p1, {u: dest[u]}), D)) // 2
+        before = (J0 - _J(g, _moved(part, {u: dest[u]}), D)) // 2
+        if gain[u] + (after - before) >= 0:
+            moves.append(u)
+    gh = sum(orc.mix64((orc.mix64(u) + (gain[u] & U64)) & U64) for u in gain) & U64
+    mh = sum(orc.mix64((u << 32) | dest[u]) for u in moves) & U64
+    return len(gain), moves, gh, mh, _J(g, _moved(part, {u: dest[u] for u in moves}), D)
+
+
+TINY = [
+    ("rgg70w", lambda: heipa.gen_rgg(70, 6.0, 3, 9), [2, 2, 2], [1, 10, 100]),
+    ("grid8x9", lambda: heipa.gen_grid(8, 9), [3, 2], [1, 7]),
+    ("rmat6w", lambda: heipa.gen_rmat(6, 0, 6.0, 0.57, 0.19, 0.19, 5, 9, True), [2, 4], [2, 5]),
+]
+
+
+@pytest.mark.parametrize("name,make,hier,dist", TINY)
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_map_lp_first_iteration_brute_force(name, make, hier, dist, seed):
+    g = make()
+    D = orc.distance_matrix(hier, dist)
+    k = D.shape[0]
+    part = random_partition(g.n, k, seed)
+    lmax = int(g.g_weight)  # never overloaded: the first iteration is LP
+    n_cand, moves, gh, mh, J1 = _first_lp_iteration(g, part, D)
+    J0 = orc.comm_cost(g, part, D)
+    _, _, _, tr = orc.map_refine(g, part, D, lmax, J0, int(block_weights(g, part, k).max()), trace_cap=64)
+    if not moves:
+        assert not tr or tr[0].iteration > 1
+        return
+    t = tr[0]
+    assert (t.iteration, t.mode) == (1, 0)
+    assert (t.n_cand, t.n_moves, t.gain_hash, t.moves_hash, t.cut) == (n_cand, len(moves), gh, mh, J1)
+
+
+GRAPHS = [
+    ("rgg3k", lambda: heipa.gen_rgg(3000, 10.0, 2, 10)),
+    ("grid50", lambda: heipa.gen_grid(50, 50)),
+    ("rmat11", lambda: heipa.gen_rmat(11, 0, 8.0, 0.57, 0.19, 0.19, 9, 20, True)),
+]
+
+
+@pytest.mark.parametrize("name,make", GRAPHS)
+@pytest.mark.parametrize("hier,dist,skew", [([4, 3], [1, 10], 0.0), ([2, 2, 3], [1, 5, 40], 0.7),
+                                            ([12], [3], 0.3)])
+def test_map_refine_invariants(name, make, hier, dist, skew):
+    g = make()
+    D = orc.distance_matrix(hier, dist)
+    k = D.shape[0]
+    lmax = math.ceil(1.03 * g.g_weight / k)
+    part0 = random_partition(g.n, k, 4, skew)
+    J0, mw0 = orc.comm_cost(g, part0, D), int(block_weights(g, part0, k).max())
+    p, J, mw, tr = orc.map_refine(g, part0, D, lmax, J0, mw0, trace_cap=100000)
+    assert J == orc.comm_cost(g, p, D) == _J(g, p, D)  # incremental J == recount
+    assert mw == block_weights(g, p, k).max()
+    assert p.max() < k
+    if mw0 <= lmax:
+        assert J <= J0
+    for t in tr:
+        assert t.n_moves > 0 and t.mode in (0, 1, 2) and 1 <= t.iteration <= 12
+    # one weak iteration per rebalancing run, then strong (promap:1265-1271):
+    # a weak iteration with moves directly follows an LP iteration or starts
+    # the run, never another rebalance iteration
+    for a, b in zip(tr, tr[1:]):
+        if b.mode == 1:
+            assert a.mode == 0
+    p2, J2, _, _ = orc.map_refine(g, part0, D, lmax, J0, mw0)  # deterministic
+    assert np.array_equal(p, p2) and J == J2
+
+
+def test_map_refine_uniform_distance_is_twice_cut():
+    g = heipa.gen_rgg(2000, 9.0, 5, 10)
+    k = 7
+    D = orc.distance_matrix([k], [1])
+    lmax = math.ceil(1.03 * g.g_weight / k)
+    part0 = random_partition(g.n, k, 2, 0.4)
+    p, J, _, _ = orc.map_refine(g, part0, D, lmax, orc.comm_cost(g, part0, D),
+                                int(block_weights(g, part0, k).max()))
+    assert J == 2 * orc.edge_cut(g, p)
+
+
+def _gms_python(g, hier, eps, seed):
+    """global_multisection_host (global_multisection.h:139-259), restated again."""
+    l = len(hier)
+    gk = int(np.prod(hier))
+    index_vec = [1]
+    for i in range(l - 1):
+        index_vec.append(index_vec[i] * hier[i])
+    k_rem, p = [], 1
+    for h in hier:
+        k_rem.append(p * h)
+        p *= h
+    out = np.zeros(g.n, np.uint32)
+    W = int(g.g_weight)
+
+    def rec(sg, n_to_o, level, ident):
+        k = hier[level]
+        lw = int(sg.vwgt.astype(np.int64).sum()) if sg.vwgt is not None else sg.n
+        depth = l - len(ident)
+        imb = ((1.0 + eps) * (k_rem[l - 1 - len(ident)] * W) / (gk * lw)) ** (1.0 / depth) - 1.0
+        tmp = heipa.initial_partition(sg, k, imb, seed)
+        if len(ident) == l - 1:
+            off = sum(ident[i] * index_vec[len(index_vec) - 1 - i] for i in range(l - 1))
+            out[n_to_o] = off + tmp
+            return
+        for b in range(k):
+            verts = np.nonzero(tmp == b)[0]
+            if len(verts) == 0:
+                continue
+            loc = {int(v): i for i, v in enumerate(verts)}
+            xadj, adj, ew = [0], [], []
+            for v in verts:
+                for j in range(sg.xadj[v], sg.xadj[v + 1]):
+                    t = int(sg.adjncy[j])
+                    if t in loc:
+                        adj.append(loc[t])
+                        ew.append(1 if sg.adjwgt is None else int(sg.adjwgt[j]))
+                xadj.append(len(adj))
+            vw = np.ones(len(verts), np.int32) if sg.vwgt is None else sg.vwgt[verts]
+            child = heipa.HostGraph(np.array(xadj, np.uint32), np.array(adj, np.uint32), np.array(ew, np.int32),
+                                    np.asarray(vw, np.int32))
+            rec(child, n_to_o[verts], level - 1, ident + [b])
+
+    rec(g, np.arange(g.n), l - 1, [])
+    return out
+
+
+@pytest.mark.parametrize("hier", [[4, 3], [2, 3, 2], [3, 1, 2], [5]])
+def test_global_multisection_python_restatement(hier):
+    g = heipa.gen_rgg(1500, 8.0, 11, 10)
+    a = orc.global_multisection(g, hier, 0.03, 7)
+    b = _gms_python(g, hier, 0.03, 7)
+    assert np.array_equal(a, b)
+    assert a.max() < np.prod(hier)
+
+
+@pytest.mark.parametrize("kind,hier,dist", [("rgg", [4, 8, 6], [1, 10, 100]), ("grid", [2, 2, 2], [1, 4, 16]),
+                                            ("rmat", [3, 4], [1, 9])])
+def test_im_partition_invariants(kind, hier, dist):
+    g = {"grid": lambda: heipa.gen_grid(60, 70), "rgg": lambda: heipa.gen_rgg(20000, 10.0, 1, 10),
+         "rmat": lambda: heipa.gen_rmat(11, 0, 8.0, 0.57, 0.19, 0.19, 9, 20, True)}[kind]()
+    k = int(np.prod(hier))
+    D = orc.distance_matrix(hier, dist)
+    p, J, mw, lv = orc.im_partition(g, hier, dist, 0.03, 1)
+    assert p.max() < k
+    assert J == orc.comm_cost(g, p, D)
+    assert mw == block_weights(g, p, k).max()
+    assert lv[-1].n <= 8 * k and all(a.n > 8 * k for a in lv[:-1])  # promap_solver.h:220-221
+    assert lv[0].cut == J
+    # refinement never worsens J at a balanced level
+    for fine, coarse in zip(lv, lv[1:]):
+        if coarse.maxw <= math.ceil(1.03 * g.g_weight / k):
+            assert fine.cut <= coarse.cut
+    p2, J2, _, _ = orc.im_partition(g, hier, dist, 0.03, 1)
+    assert np.array_equal(p, p2) and J == J2
