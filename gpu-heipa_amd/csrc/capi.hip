@@ -280,7 +280,7 @@ int heipa_project(heipa_ctx *ctx, const uint32_t *mapping, uint32_t fine_n, cons
 
 static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
                       uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace, uint32_t trace_cap,
-                      uint32_t *trace_len) {
+                      uint32_t *trace_len, const int32_t *D = nullptr) {
     return guarded(ctx, [&](Ctx &c) {
         check_graph(g);
         require(cut && maxw && k > 0 && (part || g->n == 0), "null argument");
@@ -290,7 +290,8 @@ static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint
         BackScope bs(c.arena);
         GraphMeta meta = make_meta(c, d, true);
         std::vector<heipa_jet_trace> tr;
-        JetResult r = jet_refine(c, d, meta, part, k, lmax, ultra != 0, level, *cut, *maxw, trace ? &tr : nullptr);
+        JetResult r =
+            jet_refine(c, d, meta, part, k, lmax, ultra != 0, level, *cut, *maxw, trace ? &tr : nullptr, D);
         *cut = r.cut;
         *maxw = r.maxw;
         if (trace) {
@@ -310,6 +311,68 @@ int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part
                             int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
                             uint32_t trace_cap, uint32_t *trace_len) {
     return jet_common(ctx, g, part, k, lmax, ultra, level, cut, maxw, trace, trace_cap, trace_len);
+}
+
+int heipa_map_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
+                     int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw) {
+    if (!D) return guarded(ctx, [](Ctx &) { require(false, "null distance matrix"); });
+    return jet_common(ctx, g, part, k, lmax, 0, level, J, maxw, nullptr, 0, nullptr, D);
+}
+
+int heipa_map_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
+                            int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw, heipa_jet_trace *trace,
+                            uint32_t trace_cap, uint32_t *trace_len) {
+    if (!D) return guarded(ctx, [](Ctx &) { require(false, "null distance matrix"); });
+    return jet_common(ctx, g, part, k, lmax, 0, level, J, maxw, trace, trace_cap, trace_len, D);
+}
+
+static void fill_stats(heipa_solve_stats *stats, const SolveResult &r) {
+    if (!stats) return;
+    stats->cut = r.cut;
+    stats->maxw = r.maxw;
+    stats->lmax = r.lmax;
+    stats->initial_cut = r.initial_cut;
+    stats->levels = r.levels;
+    stats->jet_iterations = r.jet_iterations;
+    stats->ms_coarsening = r.ms[0];
+    stats->ms_contraction = r.ms[1];
+    stats->ms_initial = r.ms[2];
+    stats->ms_uncontraction = r.ms[3];
+    stats->ms_refinement = r.ms[4];
+}
+
+int heipa_im_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hier, const int32_t *dist, uint32_t l,
+                       double imbalance, uint64_t seed, uint32_t *part, heipa_solve_stats *stats,
+                       heipa_level_trace *levels, uint32_t level_cap) {
+    return guarded(ctx, [&](Ctx &c) {
+        check_graph(g);
+        require(hier && dist && l > 0 && (part || g->n == 0), "bad argument");
+        require(imbalance >= 0.0, "imbalance must be non-negative");
+        MapSpec ms;
+        ms.hier.assign(hier, hier + l);
+        u64 k = 1;
+        for (u32 x: ms.hier) {
+            require(x > 0, "hierarchy entries must be positive");
+            k *= x;
+            require(k <= (1u << 16), "hierarchy product too large");
+        }
+        DGraph d = to_dgraph(g);
+        validate_graph(c, d);
+        std::vector<s32> Dh(k * k);
+        require(heipa_distance_matrix(hier, dist, l, Dh.data()) == HEIPA_OK, "bad hierarchy");
+        const size_t depth0 = c.arena.front_depth();
+        s32 *Dd = (s32 *) c.arena.push_front(4ull * k * k);
+        copy_h2d(c, Dd, Dh.data(), 4ull * k * k);
+        ms.D = Dd;
+        std::vector<heipa_level_trace> lv;
+        SolveResult r = solve(c, d, (u32) k, imbalance, seed, false, part, levels ? &lv : nullptr, &ms);
+        c.sync();
+        c.arena.release_front(depth0);
+        fill_stats(stats, r);
+        if (levels) {
+            for (u32 i = 0; i < lv.size() && i < level_cap; ++i) levels[i] = lv[i];
+        }
+    });
 }
 
 int heipa_partition(heipa_ctx *ctx, const heipa_graph *g, uint32_t k, double imbalance, uint64_t seed, int ultra,
@@ -519,6 +582,18 @@ int heipa_initial_partition(uint32_t n, const uint32_t *xadj, const uint32_t *ad
     if (!xadj || !part || k == 0) return HEIPA_ERR_INVALID;
     return guarded_host([&]() {
         initial_partition(n, xadj, adjncy, adjwgt, vwgt, k, imbalance, seed, part);
+        return HEIPA_OK;
+    });
+}
+
+int heipa_global_multisection(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
+                              const int32_t *vwgt, const uint32_t *hier, uint32_t l, double imbalance, uint64_t seed,
+                              uint32_t *part) {
+    if (!xadj || (!part && n) || !hier || l == 0) return HEIPA_ERR_INVALID;
+    for (u32 i = 0; i < l; ++i)
+        if (hier[i] == 0) return HEIPA_ERR_INVALID;
+    return guarded_host([&]() {
+        global_multisection(n, xadj, adjncy, adjwgt, vwgt, hier, l, imbalance, seed, part);
         return HEIPA_OK;
     });
 }

@@ -33,8 +33,11 @@ struct JetResult {
     s64 cut = 0, maxw = 0;
     u32 iterations = 0;
 };
+// D (device, k x k, row-major) selects the mapping objective: `cut` is then
+// the comm cost J and the loop is ProMap_jet_refine's
+// (src/refinement/promap_jet_label_propagation.h:1219-1311).
 JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, u32 k, s64 lmax, bool ultra,
-                     u32 level, s64 cut, s64 maxw, std::vector<heipa_jet_trace> *trace);
+                     u32 level, s64 cut, s64 maxw, std::vector<heipa_jet_trace> *trace, const s32 *D = nullptr);
 
 // ---- drivers (solver.hip)
 struct SolveResult {
@@ -42,10 +45,16 @@ struct SolveResult {
     u32 levels = 0, jet_iterations = 0;
     double ms[5] = {0, 0, 0, 0, 0};
 };
+// The IM process-mapping objective: hierarchy and the device distance matrix
+// (k x k with k = prod(hier)).
+struct MapSpec {
+    std::vector<u32> hier;
+    const s32 *D = nullptr;
+};
+// map != null: ProMapSolver::solve (IM); cut / initial_cut / level cuts hold J
 SolveResult solve(Ctx &c, const DGraph &g, u32 k, double imbalance, u64 seed, bool ultra, u32 *part,
-                  std::vector<heipa_level_trace> *levels);
+                  std::vector<heipa_level_trace> *levels, const MapSpec *map = nullptr);
 void hm_partition(Ctx &c, const DGraph &g, const std::vector<u32> &hier, double imbalance, u64 seed, bool ultra,
                   u32 *part);
-double adaptive_imbalance(double eps, s64 gW, u32 gk, s64 lW, u32 krem, u64 depth);
 
 } // namespace heipa

@@ -128,6 +128,7 @@ struct JetCtl {
     u32 save;         // the partition after the last iteration is the best so far
     u32 iters;        // iterations that changed the partition
     u32 n_trace, stamp;
+    u32 zero_lp;      // mapping objective: LP iterations in a row without moves
 };
 
 struct JetDev {
@@ -160,6 +161,7 @@ struct JetDev {
     s64 *evict_adjust;  // k
     s64 *evict_start;   // k + 1
     s64 *counter;       // strong-rebalance overflow position
+    const s32 *D;       // k x k distances: mapping objective J (ProMap); null: edge cut
 };
 
 __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64) d.vw[u] : 1; }
@@ -324,15 +326,41 @@ __device__ __forceinline__ T group_sum(T x, T *red) {
 // the group writes the results; returns whether u is a candidate
 template<int G>
 __device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, double ratio,
-                                        Best *red, bool active) {
+                                        Best *red, s64 *red2, bool active) {
     const u32 T = active ? *gc.tcount : 0;
     const u32 own_b = active ? d.work[u] : 0;
     Best best{0, NO_MOVE};
-    for (u32 i = lane; i < T; i += G) {
-        const u32 b = gc.touched[i];
-        if (b == own_b) continue;
-        Best x{(s64) gc.cnt[b], b};
-        if (best_better(x, best)) best = x;
+    if (d.D) {
+        // ProMap_jet_lp best_block (promap_jet_label_propagation.h:230-258):
+        // gain_delta (:180-204) of a move to b is J(own) - J(b) with
+        // J(x) = sum over the touched blocks c of conn(c) * D(x, c); the lanes
+        // split the candidate blocks, each summing over all touched blocks
+        const s32 *Do = d.D + (u64) own_b * d.k;
+        s64 jo = 0;
+        for (u32 i = lane; i < T; i += G) {
+            const u32 c = gc.touched[i];
+            jo += (s64) gc.cnt[c] * (s64) Do[c];
+        }
+        jo = group_sum<G>(jo, red2);
+        for (u32 i = lane; i < T; i += G) {
+            const u32 b = gc.touched[i];
+            if (b == own_b) continue;
+            const s32 *Db = d.D + (u64) b * d.k;
+            s64 jb = 0;
+            for (u32 j = 0; j < T; ++j) {
+                const u32 c = gc.touched[j];
+                jb += (s64) gc.cnt[c] * (s64) Db[c];
+            }
+            Best x{jo - jb, b};
+            if (best_better(x, best)) best = x;
+        }
+    } else {
+        for (u32 i = lane; i < T; i += G) {
+            const u32 b = gc.touched[i];
+            if (b == own_b) continue;
+            Best x{(s64) gc.cnt[b], b};
+            if (best_better(x, best)) best = x;
+        }
     }
     best = best_reduce<G>(best, red);
     bool cand = false;
@@ -343,8 +371,14 @@ __device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u3
         s64 gain = 0;
         if (best_id != NO_MOVE) {
             const s64 bc = best.c;
-            if (bc >= own || ((double) own - (double) bc) < floor(ratio * (double) own)) gain = bc - own;
-            else best_id = NO_MOVE;
+            if (d.D) {
+                if (bc >= 0) gain = bc;  // promap:232: best_delta starts at -1
+                else best_id = NO_MOVE;
+            } else if (bc >= own || ((double) own - (double) bc) < floor(ratio * (double) own)) {
+                gain = bc - own;
+            } else {
+                best_id = NO_MOVE;
+            }
         }
         d.gcache[u] = gain;
         d.dcache[u] = best_id;
@@ -587,7 +621,7 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
                 conn_accumulate(d, gc, u, lane, G);
             }
             wave_sync();
-            const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, active)
+            const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, nullptr, active)
                                           : reb_eval<G>(d, strong, mbw, rs, gc, u, lane, nullptr, nullptr, active);
             wave_sync();
             if (active) conn_reset(gc, *gc.tcount, lane, G);
@@ -676,7 +710,7 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     __syncthreads();
     conn_accumulate(d, gc, u, threadIdx.x, JB);
     __syncthreads();
-    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, ctl->ratio, red, true)
+    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, ctl->ratio, red, red2, true)
                                   : reb_eval<JB>(d, strong, mbw, rs, gc, u, threadIdx.x, red, red2, true);
     append(listed, u, d.cand_x, &ctl->st[ST_CAND]);
     if (threadIdx.x == 0) {
@@ -692,6 +726,19 @@ __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 la
     const s64 ug = d.gain1[u];
     const u32 o = d.work[u], nn = d.dest[u];
     s64 change = 0;
+    if (d.D) {
+        // promap:297-314: the gain of u once v has moved minus its gain before
+        const s32 *Do = d.D + (u64) o * d.k, *Dn = d.D + (u64) nn * d.k;
+        for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
+            const u32 v = d.adj[j];
+            const s64 vg = d.gain1[v];
+            const bool first = vg > ug || (vg == ug && v < u);
+            if (!first) continue;
+            const u32 vn = d.dest[v], vo = d.work[v];
+            change += ewt(d, j) * (((s64) Do[vn] - (s64) Dn[vn]) - ((s64) Do[vo] - (s64) Dn[vo]));
+        }
+        return change;
+    }
     for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
         const u32 v = d.adj[j];
         const s64 vg = d.gain1[v];
@@ -1113,6 +1160,25 @@ __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
 __device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 stride, u32 stamp) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
     s64 delta = 0;
+    if (d.D) {
+        // comm cost change (promap:1174-1207): moves that keep their block
+        // contribute nothing; an edge to a vertex that did not change block
+        // counts twice (its other direction is not visited)
+        const s32 *Dn = d.D + (u64) nu * d.k, *Do = d.D + (u64) ou * d.k;
+        for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
+            const u32 v = d.adj[j];
+            const bool mv = d.mark[v] == stamp;
+            const u32 ov = mv ? d.oldp[v] : d.work[v];
+            const u32 nv = mv ? d.dest[v] : ov;
+            if (ou != nu) {
+                const s64 x = ewt(d, j) * ((s64) Dn[nv] - (s64) Do[ov]);
+                delta += nv != ov ? x : 2 * x;
+            }
+            d.dcache[v] = NULL_PART;
+            d.icache[v] = -1;
+        }
+        return delta;
+    }
     for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
         const u32 v = d.adj[j];
         const bool mv = d.mark[v] == stamp;
@@ -1170,6 +1236,79 @@ __device__ void account(const JetDev &d, JetCtl *ctl, const unsigned long long *
     ctl->bytes[4] += 16.0 * nmv + 4.0 * w * emv + fmin(16.0 * emv, 16.0 * n);
 }
 
+__device__ void trace_record(const JetDev &d, JetCtl *ctl, const unsigned long long *rep, u32 mode, s64 cur,
+                             s64 cur_maxw) {
+    heipa_jet_trace &t = d.trace[ctl->n_trace % TRACE_RING];
+    t.ratio_idx = ctl->ri;
+    t.iteration = ctl->it;
+    t.mode = mode;
+    t.n_cand = (u32) rep[ST_CAND];
+    t.n_moves = (u32) rep[ST_MOVES];
+    t.pad = 0;
+    t.cut = cur;
+    t.maxw = cur_maxw;
+    t.moves_hash = rep[ST_MHASH];
+    t.gain_hash = mode == 0 ? rep[ST_GHASH] : 0;
+    ctl->n_trace++;
+}
+
+// One step of ProMap_jet_refine's loop (promap_jet_label_propagation.h:
+// 1254-1305) for the mapping objective: an iteration without moves is
+// counted and skipped; the balance counter is raised before every rebalance
+// iteration (one weak, then strong); the second best-partition test is
+// strict on the max block weight.  The loop ends early where it can only
+// repeat itself: after a strong iteration without moves (nothing it reads
+// changed) or after two LP iterations in a row without moves (the first
+// clears every lock, so the second and all later ones see the same state);
+// the remaining iterations of the reference change nothing.
+__device__ void map_control(const JetDev &d, JetCtl *ctl, const unsigned long long *rep, s64 maxw) {
+    const u32 mode = ctl->mode;
+    const u32 n_moves = (u32) rep[ST_MOVES];
+    u32 bal = ctl->bal, it = ctl->it;
+    u32 save = 0;
+    bool done = false;
+    if (n_moves) {
+        ctl->iters++;
+        const s64 cur = ctl->cur_cut + (s64) rep[ST_DELTA];
+        ctl->cur_cut = cur;
+        ctl->cur_maxw = maxw;
+        if (d.trace) trace_record(d, ctl, rep, mode, cur, maxw);
+        const s64 best = ctl->best_cut, best_maxw = ctl->best_maxw;
+        if (best_maxw > d.lmax && maxw < best_maxw) {
+            save = 1;
+            ctl->best_cut = cur;
+            ctl->best_maxw = maxw;
+            it = 0;
+        } else if (cur < best && (maxw <= d.lmax || maxw < best_maxw)) {
+            if ((double) cur < PHI * (double) best) it = 0;
+            save = 1;
+            ctl->best_cut = cur;
+            ctl->best_maxw = maxw;
+        }
+        ctl->zero_lp = 0;
+    } else if (mode == 0) {
+        done = ++ctl->zero_lp >= 2;
+    } else {
+        done = mode == 2;
+    }
+    ctl->save = save;
+    if (done || it >= N_MAX_ITERATIONS) {
+        ctl->active = 0;
+    } else {
+        it += 1;
+        if (ctl->cur_maxw <= d.lmax) {
+            bal = 0;
+            ctl->mode = 0;
+        } else {
+            bal++;
+            ctl->mode = bal < N_MAX_WEAK_ITERATIONS ? 1u : 2u;
+        }
+    }
+    ctl->it = it;
+    ctl->bal = bal;
+    ctl->stamp++;
+}
+
 // One step of jet_refine's loop (jet:960-1030) after an iteration: counters,
 // cut and block weight, best-partition tracking, ratio and mode of the next
 // iteration.  Runs on one thread of the control launch.
@@ -1194,6 +1333,11 @@ __device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot) {
     const u32 mode = ctl->mode;
     const u32 n_moves = (u32) rep[ST_MOVES];
     account(d, ctl, rep, mode, ctl->save != 0);
+    if (d.D) {
+        map_control(d, ctl, rep, maxw);
+        *d.ctl = cl;
+        return;
+    }
     u32 bal = ctl->bal, it = ctl->it, ri = ctl->ri;
     bool counted = true, end_ratio = false;
     if (mode == 0) {
@@ -1217,20 +1361,7 @@ __device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot) {
         const s64 cur_maxw = maxw;
         ctl->cur_cut = cur_cut;
         ctl->cur_maxw = cur_maxw;
-        if (d.trace) {
-            heipa_jet_trace &t = d.trace[ctl->n_trace % TRACE_RING];
-            t.ratio_idx = ri;
-            t.iteration = it;
-            t.mode = mode;
-            t.n_cand = (u32) rep[ST_CAND];
-            t.n_moves = n_moves;
-            t.pad = 0;
-            t.cut = cur_cut;
-            t.maxw = cur_maxw;
-            t.moves_hash = rep[ST_MHASH];
-            t.gain_hash = mode == 0 ? rep[ST_GHASH] : 0;
-            ctl->n_trace++;
-        }
+        if (d.trace) trace_record(d, ctl, rep, mode, cur_cut, cur_maxw);
         const s64 best_cut = ctl->best_cut, best_maxw = ctl->best_maxw;
         if (best_maxw > d.lmax && cur_maxw < best_maxw) {
             save = 1;
@@ -1479,7 +1610,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
                     conn_accumulate(d, gc, u, lane, G);
                 }
                 wave_sync();
-                const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, active)
+                const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, nullptr, active)
                                               : reb_eval<G>(d, strong, mbw, rs, gc, u, lane, nullptr, nullptr, active);
                 wave_sync();
                 if (active) conn_reset(gc, *gc.tcount, lane, G);
@@ -1633,8 +1764,10 @@ __global__ void ctl_init_kernel(JetDev d, RatioTable rt, s64 cut, s64 maxw) {
     for (u32 i = 0; i < 5; ++i) ctl->bytes[i] = 0.0;
     ctl->ri = 0;
     ctl->it = 1;
-    ctl->bal = 0;
+    // the mapping loop raises its balance counter before a rebalance iteration
+    ctl->bal = (d.D && maxw > d.lmax) ? 1u : 0u;
     ctl->mode = maxw <= d.lmax ? 0u : 1u;
+    ctl->zero_lp = 0;
     ctl->active = 1;
     ctl->stalled = 0;
     ctl->save = 0;
@@ -1843,7 +1976,7 @@ struct JetHost {
 } // namespace
 
 JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, u32 k, s64 lmax, bool ultra,
-                     u32 level, s64 cut, s64 maxw, std::vector<heipa_jet_trace> *trace) {
+                     u32 level, s64 cut, s64 maxw, std::vector<heipa_jet_trace> *trace, const s32 *D) {
     JetResult res;
     res.cut = cut;
     res.maxw = maxw;
@@ -1861,6 +1994,8 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.ew = g.ew;
     d.vw = g.vw;
     d.hub_deg = meta.hub_deg;
+    d.D = D;
+    if (D) ultra = false;  // ProMap_jet_refine runs one filter ratio, which its LP does not read
     d.part = part;
     d.work = bs.get<u32>(n);
     d.dest = bs.get<u32>(n);

@@ -1,4 +1,10 @@
-// Multilevel drivers: the plain Solver level loop and the HM recursion.
+// Multilevel drivers: the plain Solver level loop, the IM process-mapping
+// solver and the HM recursion.
+//
+// IM (src/datastructures/promap_solver.h:146-347, --config IM): the same
+// level loop with global multisection of the coarsest graph
+// (global_multisection.h:139-287, host), the comm cost J as the tracked
+// objective and the mapping form of Jet at every finer level.
 //
 // Solver (src/datastructures/solver.h:144-246 device ctor, :340-556): coarsen
 // while n > 8k (two-hop matching with seed 12345, then contraction), partition
@@ -222,15 +228,11 @@ void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u
 
 } // namespace
 
-double adaptive_imbalance(double eps, s64 gW, u32 gk, s64 lW, u32 krem, u64 depth) {
-    double li = (1.0 + eps) * ((double) ((u64) krem * (u64) gW) / (double) ((u64) gk * (u64) lW));
-    return std::pow(li, 1.0 / (double) depth) - 1.0;
-}
-
 SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, bool ultra, u32 *part,
-                  std::vector<heipa_level_trace> *levels) {
+                  std::vector<heipa_level_trace> *levels, const MapSpec *map) {
     SolveResult res;
     require(k >= 1, "k must be >= 1");
+    if (map) ultra = false;
     res.lmax = (s64) std::ceil((1.0 + imbalance) * ((double) g0.gw / (double) k));
     const size_t depth0 = c.arena.front_depth();
     const u32 n0 = g0.n;
@@ -273,12 +275,16 @@ SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, b
             hv.resize(cg.n);
             readback(c, hv.data(), cg.vw, cg.n);
         }
-        initial_partition(cg.n, hx.data(), ha.data(), cg.ew ? hw.data() : nullptr, cg.vw ? hv.data() : nullptr, k,
-                          imbalance, seed, hp.data());
+        if (map)  // promap_solver.h:336-347
+            global_multisection(cg.n, hx.data(), ha.data(), cg.ew ? hw.data() : nullptr, cg.vw ? hv.data() : nullptr,
+                                map->hier.data(), (u32) map->hier.size(), imbalance, seed, hp.data());
+        else
+            initial_partition(cg.n, hx.data(), ha.data(), cg.ew ? hw.data() : nullptr, cg.vw ? hv.data() : nullptr,
+                              k, imbalance, seed, hp.data());
         copy_h2d(c, pa, hp.data(), 4ull * cg.n);
         c.sync();
     }
-    s64 cut = edge_cut(c, cg, metas.back(), pa);
+    s64 cut = map ? comm_cost(c, cg, metas.back(), pa, map->D, k) : edge_cut(c, cg, metas.back(), pa);
     s64 maxw = max_block_weight(c, cg, pa, k);
     res.initial_cut = cut;
     res.ms[2] += now_ms() - ti;
@@ -302,7 +308,8 @@ SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, b
         c.sync();
         double tb = now_ms();
         res.ms[3] += tb - ta;
-        JetResult jr = jet_refine(c, graphs[l], metas[l], cur, k, res.lmax, ultra, (u32) l, cut, maxw, nullptr);
+        JetResult jr = jet_refine(c, graphs[l], metas[l], cur, k, res.lmax, ultra, (u32) l, cut, maxw, nullptr,
+                                  map ? map->D : nullptr);
         cut = jr.cut;
         maxw = jr.maxw;
         res.jet_iterations += jr.iterations;

@@ -109,4 +109,13 @@ inline s32 edge_weight_hash(u64 seed, u32 u, u32 v, u32 wmod) {
 void initial_partition(u32 n, const u32 *xadj, const u32 *adjncy, const s32 *adjwgt,
                        const s32 *vwgt, u32 k, double imbalance, u64 seed, u32 *part);
 
+// ---- Global multisection (host, serial): the IM path's initial mapping
+// (src/initial_partitioning/global_multisection.h:139-287) over
+// initial_partition; part receives ids < prod(hier).  Throws
+// std::invalid_argument on an empty hierarchy or a zero entry.
+void global_multisection(u32 n, const u32 *xadj, const u32 *adjncy, const s32 *adjwgt, const s32 *vwgt,
+                         const u32 *hier, u32 l, double imbalance, u64 seed, u32 *part);
+// determine_adaptive_imbalance (global_multisection.h:39-48)
+double adaptive_imbalance(double eps, s64 gW, u32 gk, s64 lW, u32 krem, u64 depth);
+
 } // namespace heipa

@@ -124,6 +124,12 @@ def lib():
             "heipa_partition": (c_int, [vp, GP, c_u32, c_dbl, c_u64, c_int, vp, ctypes.POINTER(SolveStats),
                                         ctypes.POINTER(LevelTrace), c_u32]),
             "heipa_hm_partition": (c_int, [vp, GP, u32p, c_u32, c_dbl, c_u64, c_int, vp]),
+            "heipa_map_refine": (c_int, [vp, GP, vp, vp, c_u32, ctypes.c_int64, c_u32, s64p, s64p]),
+            "heipa_map_refine_traced": (c_int, [vp, GP, vp, vp, c_u32, ctypes.c_int64, c_u32, s64p, s64p,
+                                                ctypes.POINTER(JetTrace), c_u32, u32p]),
+            "heipa_im_partition": (c_int, [vp, GP, u32p, s32p, c_u32, c_dbl, c_u64, vp, ctypes.POINTER(SolveStats),
+                                           ctypes.POINTER(LevelTrace), c_u32]),
+            "heipa_global_multisection": (c_int, [c_u32, u32p, u32p, s32p, s32p, u32p, c_u32, c_dbl, c_u64, u32p]),
             "heipa_adaptive_imbalance": (c_dbl, [c_dbl, ctypes.c_int64, c_u32, ctypes.c_int64, c_u32, c_u64]),
             "heipa_ctx_set_option": (c_int, [vp, c_int, c_u64]),
             "heipa_pool_create": (c_int, [c_int, c_u32, c_u64, c_u64, ctypes.POINTER(vp)]),
@@ -297,6 +303,23 @@ def initial_partition(g: HostGraph, k, imbalance, seed):
     return part
 
 
+def global_multisection(g: HostGraph, hier, imbalance, seed):
+    """The IM path's initial mapping (host): ids < prod(hier)."""
+    part = np.zeros(max(g.n, 1), np.uint32)
+    xadj = np.ascontiguousarray(g.xadj, np.uint32)
+    adjncy = np.ascontiguousarray(g.adjncy, np.uint32)
+    ew = None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32)
+    vw = None if g.vwgt is None else np.ascontiguousarray(g.vwgt, np.int32)
+    h = np.ascontiguousarray(hier, np.uint32)
+    rc = lib().heipa_global_multisection(g.n, _ptr(xadj, ctypes.c_uint32), _ptr(adjncy, ctypes.c_uint32),
+                                         None if ew is None else _ptr(ew, ctypes.c_int32),
+                                         None if vw is None else _ptr(vw, ctypes.c_int32), _ptr(h, ctypes.c_uint32),
+                                         len(h), imbalance, seed, _ptr(part, ctypes.c_uint32))
+    if rc != OK:
+        raise HeipaError("heipa_global_multisection", rc)
+    return part[:g.n]
+
+
 def adaptive_imbalance(eps, gW, gk, lW, krem, depth):
     return lib().heipa_adaptive_imbalance(eps, gW, gk, lW, krem, depth)
 
@@ -430,6 +453,38 @@ class Context:
         self._check("heipa_partition", self.L.heipa_partition(self.h, ctypes.byref(g.desc), k, imbalance, seed,
                                                               int(ultra), part.data_ptr(), ctypes.byref(st), lv,
                                                               level_cap))
+        levels = [lv[i] for i in range(min(st.levels, level_cap))] if level_cap else None
+        return part[:g.n], st, levels
+
+    def map_refine(self, g: DeviceGraph, part, D, k, lmax, level, J, maxw, trace_cap=0):
+        """ProMap_jet_refine on the device: D is a device int32 k x k tensor; returns (J, maxw, trace)."""
+        self._sync()
+        j, mw = ctypes.c_int64(J), ctypes.c_int64(maxw)
+        if trace_cap:
+            tr = (JetTrace * trace_cap)()
+            tl = ctypes.c_uint32()
+            self._check("heipa_map_refine_traced", self.L.heipa_map_refine_traced(
+                self.h, ctypes.byref(g.desc), part.data_ptr(), D.data_ptr(), k, lmax, level, ctypes.byref(j),
+                ctypes.byref(mw), tr, trace_cap, ctypes.byref(tl)))
+            return j.value, mw.value, [tr[i] for i in range(min(tl.value, trace_cap))]
+        self._check("heipa_map_refine", self.L.heipa_map_refine(self.h, ctypes.byref(g.desc), part.data_ptr(),
+                                                                D.data_ptr(), k, lmax, level, ctypes.byref(j),
+                                                                ctypes.byref(mw)))
+        return j.value, mw.value, None
+
+    def im_partition(self, g: DeviceGraph, hier, dist, imbalance=0.03, seed=1, part=None, level_cap=0):
+        """ProMapSolver::solve (--config IM); returns (part, stats, levels) with J in the cut fields."""
+        torch = _torch()
+        if part is None:
+            part = torch.empty(max(g.n, 1), dtype=torch.int32, device=g.xadj.device)
+        h = np.ascontiguousarray(hier, np.uint32)
+        d = np.ascontiguousarray(dist, np.int32)
+        st = SolveStats()
+        lv = (LevelTrace * level_cap)() if level_cap else None
+        self._sync()
+        self._check("heipa_im_partition", self.L.heipa_im_partition(
+            self.h, ctypes.byref(g.desc), _ptr(h, ctypes.c_uint32), _ptr(d, ctypes.c_int32), len(h), imbalance, seed,
+            part.data_ptr(), ctypes.byref(st), lv, level_cap))
         levels = [lv[i] for i in range(min(st.levels, level_cap))] if level_cap else None
         return part[:g.n], st, levels
 

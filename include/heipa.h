@@ -134,6 +134,24 @@ int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part
                             int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
                             uint32_t trace_cap, uint32_t *trace_len);
 
+/* ---- IM process mapping (mapping objective J = sum over directed edges w(u,v) * D(part(u), part(v))) */
+/* ProMap_jet_refine<uvw,uew> — src/refinement/promap_jet_label_propagation.h:1219-1311.  D: device, k x k
+ * row-major distances (heipa_distance_matrix, uploaded).  part in/out (device); J and maxw in/out (current
+ * values on entry, the best partition's on return).  level is accepted for parity with the reference's
+ * signature; the mapping LP reads no filter ratio. */
+int heipa_map_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
+                     int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw);
+/* same, recording one heipa_jet_trace per iteration that moved vertices (cut fields hold J) */
+int heipa_map_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
+                            int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw, heipa_jet_trace *trace,
+                            uint32_t trace_cap, uint32_t *trace_len);
+/* ProMapSolver<DistanceOracleMatrix>(config).solve(host_g) with --config IM — src/datastructures/
+ * promap_solver.h:146-347: coarsening, global multisection, J-driven Jet per level.  part: device.
+ * stats->cut / initial_cut and the level trace cut fields hold J.  prod(hier) <= 65536. */
+int heipa_im_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hier, const int32_t *dist, uint32_t l,
+                       double imbalance, uint64_t seed, uint32_t *part, heipa_solve_stats *stats,
+                       heipa_level_trace *levels, uint32_t level_cap);
+
 /* ---- drivers */
 /* Solver(config).solve(host_g) / the device-graph Solver ctor — src/datastructures/solver.h:144-246, 248,
  * 340-556.  part: device, n entries.  stats and levels may be NULL. */
@@ -243,6 +261,11 @@ void heipa_host_graph_free(heipa_host_graph *g);
 /* the shared host initial partitioner (replaces kway_partition, kway_core.h:1548-1588); host arrays */
 int heipa_initial_partition(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
                             const int32_t *vwgt, uint32_t k, double imbalance, uint64_t seed, uint32_t *part);
+/* global_multisection_host (src/initial_partitioning/global_multisection.h:217-259) over the shared host
+ * partitioner: the IM path's initial mapping; host arrays, part receives ids < prod(hier) */
+int heipa_global_multisection(uint32_t n, const uint32_t *xadj, const uint32_t *adjncy, const int32_t *adjwgt,
+                              const int32_t *vwgt, const uint32_t *hier, uint32_t l, double imbalance, uint64_t seed,
+                              uint32_t *part);
 
 #ifdef __cplusplus
 }

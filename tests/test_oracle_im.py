@@ -215,3 +215,11 @@ def test_im_partition_invariants(kind, hier, dist):
             assert fine.cut <= coarse.cut
     p2, J2, _, _ = orc.im_partition(g, hier, dist, 0.03, 1)
     assert np.array_equal(p, p2) and J == J2
+
+
+@pytest.mark.parametrize("hier", [[4, 8, 6], [2, 3, 2], [3, 1, 2]])
+def test_product_global_multisection_matches_oracle(hier):
+    # the library's host routine (no GPU involved) against the oracle's restatement
+    for g in (heipa.gen_rgg(1500, 8.0, 11, 10), heipa.gen_grid(30, 41),
+              heipa.gen_rmat(10, 0, 8.0, 0.57, 0.19, 0.19, 4, 10, True)):
+        assert np.array_equal(heipa.global_multisection(g, hier, 0.03, 3), orc.global_multisection(g, hier, 0.03, 3))
