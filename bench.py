@@ -8,6 +8,10 @@ partition of the coarsest graph, uncoarsening with Jet refinement) of the
 device-resident graph into a device partition array.  The metric is input
 edges/s = undirected edges / wall time of one step.
 
+Workload im-rgg20-486: GPU-HeiProMap `--config IM` (multilevel process
+mapping onto the hierarchy 4:8:6 with distances 1:10:100, k = 192, eps=0.03)
+on the same graph; quality is the comm cost J.
+
 Prints one JSON line (see the `result` dict below for the fields).
 """
 import argparse
@@ -23,6 +27,9 @@ WORKLOADS = {
     "rgg20-ultra-k64": dict(kind="rgg", n=1 << 20, deg=12.0, seed=1, wmod=10, k=64, ultra=True),
     "grid512-default-k32": dict(kind="grid", rows=512, cols=512, k=32, ultra=False),
     "rgg16-ultra-k64": dict(kind="rgg", n=1 << 16, deg=12.0, seed=1, wmod=10, k=64, ultra=True),
+    # IM process mapping (GPU-HeiProMap --config IM) onto 4:8:6 with distances 1:10:100
+    "im-rgg20-486": dict(kind="rgg", n=1 << 20, deg=12.0, seed=1, wmod=10, k=192, ultra=False,
+                         hier=[4, 8, 6], dist=[1, 10, 100]),
     # batched form: independent small graphs through a worker pool
     "batched": dict(kind="batched"),
     # served form: Poisson arrivals of partition and mapping requests
@@ -80,9 +87,12 @@ def main():
     dg = heipa.DeviceGraph(g, "cuda:0")
     part = torch.empty(g.n, dtype=torch.int32, device="cuda:0")
     k, ultra = w["k"], w["ultra"]
+    hier = w.get("hier")
 
-    def step():
-        return ctx.partition(dg, k, 0.03, args.seed, ultra, part=part)
+    def step(dgraph=dg, out=part):
+        if hier:
+            return ctx.im_partition(dgraph, hier, w["dist"], 0.03, args.seed, part=out)
+        return ctx.partition(dgraph, k, 0.03, args.seed, ultra, part=out)
 
     for _ in range(args.warmup):
         step()
@@ -146,7 +156,7 @@ def main():
     torch.cuda.synchronize()
     th0 = time.perf_counter()
     dg_h = heipa.DeviceGraph(g, "cuda:0")
-    part_h, _, _ = ctx.partition(dg_h, k, 0.03, args.seed, ultra)
+    part_h, _, _ = step(dg_h, None)
     host_part = heipa.to_numpy_u32(part_h)
     th1 = time.perf_counter()
     del dg_h, part_h
@@ -158,7 +168,7 @@ def main():
     assert np.array_equal(p, host_part)
     vw = np.ones(g.n, np.int64) if g.vwgt is None else g.vwgt.astype(np.int64)
     bw = np.bincount(p, weights=vw, minlength=k)
-    quality = {"cut": int(stats.cut), "maxw": int(stats.maxw), "lmax": int(stats.lmax),
+    quality = {"comm_cost" if hier else "cut": int(stats.cut), "maxw": int(stats.maxw), "lmax": int(stats.lmax),
                "balanced": bool(bw.max() <= stats.lmax), "levels": int(stats.levels),
                "jet_iterations": int(stats.jet_iterations)}
 
@@ -167,7 +177,10 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import heipa_oracle as orc
         tc = time.perf_counter()
-        _, ccut, cmaxw, _ = orc.partition(g, k, 0.03, args.seed, ultra)
+        if hier:
+            _, ccut, cmaxw, _ = orc.im_partition(g, hier, w["dist"], 0.03, args.seed)
+        else:
+            _, ccut, cmaxw, _ = orc.partition(g, k, 0.03, args.seed, ultra)
         dt = time.perf_counter() - tc
         cpu = {"value": round(m_und / dt, 1), "unit": "input edges/s", "cores": 1, "kind": "port",
                "sample": f"the full {args.workload} workload, one solve by the serial C++ oracle "
@@ -175,7 +188,7 @@ def main():
                "same_result_as_gpu": bool(ccut == stats.cut and cmaxw == stats.maxw)}
 
     result = {
-        "metric": "input edges/sec on 1 MI355X (plain drop-in)",
+        "metric": "input edges/sec on 1 MI355X (" + ("IM process mapping" if hier else "plain drop-in") + ")",
         "value": round(value, 1),
         "unit": "input edges/s",
         "n_gpus": 1,
@@ -188,7 +201,8 @@ def main():
         "dtype": "int32/int64",
         "data": "synthetic (seeded generator)",
         "config": {"workload": args.workload, "n": g.n, "undirected_edges": m_und, "k": k,
-                   "config": "ultra" if ultra else "default", "imbalance": 0.03, "seed": args.seed,
+                   "config": "IM" if hier else ("ultra" if ultra else "default"), "imbalance": 0.03,
+                   "seed": args.seed, **({"hierarchy": hier, "distance": w["dist"]} if hier else {}),
                    "input": "device-resident CSR", "phase_ms": {
                        "coarsening": round(stats.ms_coarsening, 2), "contraction": round(stats.ms_contraction, 2),
                        "initial": round(stats.ms_initial, 2), "uncontraction": round(stats.ms_uncontraction, 2),

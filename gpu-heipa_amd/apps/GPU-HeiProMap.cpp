@@ -1,10 +1,11 @@
 // GPU-HeiProMap: hierarchical process mapping (reference: app/GPU-HeiProMap.cpp:36-165).
 //   GPU-HeiProMap --graph g.graph --hierarchy 4:8:6 --distance 1:10:100
-//                 --config HM|HM-ultra [--imbalance 0.03] [--mapping out.txt] [--seed S]
-// k = product of the hierarchy (promap_configuration.h:111-112).  HM runs the
-// hierarchical multisection (promap_solver.h:238-241); the IM config
-// (ProMap Jet refinement) is not part of this build and is rejected.
-// J(C,D,Pi) of the result is reported with 64-bit accumulation.
+//                 --config IM|HM|HM-ultra [--imbalance 0.03] [--mapping out.txt] [--seed S]
+// k = product of the hierarchy (promap_configuration.h:111-112).  IM runs the
+// multilevel process-mapping solver (promap_solver.h:146-347: coarsening,
+// global multisection, Jet over the comm cost J); HM runs the hierarchical
+// multisection (promap_solver.h:238-241).  J(C,D,Pi) of the result is
+// reported with 64-bit accumulation.
 #include "cli.hpp"
 
 int main(int argc, char **argv) {
@@ -34,8 +35,8 @@ int main(int argc, char **argv) {
     const int verbose = std::stoi(opt.get("--verbose-level"));
     const uint64_t seed = opt.is_set("--seed") ? std::stoull(opt.get("--seed")) : (uint64_t) std::random_device{}();
     const uint64_t arena = (uint64_t) (std::stod(opt.get("--arena-gb")) * (1ull << 30));
-    if (config != "HM" && config != "HM-ultra") {
-        std::cerr << "Error: Invalid config: " << config << " (supported: HM, HM-ultra)" << std::endl;
+    if (config != "IM" && config != "HM" && config != "HM-ultra") {
+        std::cerr << "Error: Invalid config: " << config << " (supported: IM, HM, HM-ultra)" << std::endl;
         return EXIT_FAILURE;
     }
     if (hier.empty() || hier.size() != dist.size()) {
@@ -63,9 +64,20 @@ int main(int argc, char **argv) {
         uint32_t *dpart = nullptr;
         int32_t *dD = nullptr;
         CLI_HIP(hipMalloc(&dpart, 4ull * (n ? n : 1)));
-        rc = heipa_hm_partition(ctx, &dg.g, hier.data(), (uint32_t) hier.size(), imbalance, seed,
-                                config == "HM-ultra", dpart);
-        if (rc != HEIPA_OK) cli::die("heipa_hm_partition", rc, ctx);
+        if (config == "IM") {
+            heipa_solve_stats st{};
+            rc = heipa_im_partition(ctx, &dg.g, hier.data(), dist.data(), (uint32_t) hier.size(), imbalance, seed,
+                                    dpart, &st, nullptr, 0);
+            if (rc != HEIPA_OK) cli::die("heipa_im_partition", rc, ctx);
+            if (verbose >= 1) {
+                std::cout << "Init. comm-cost   : " << st.initial_cut << std::endl;
+                std::cout << "Levels            : " << st.levels << std::endl;
+            }
+        } else {
+            rc = heipa_hm_partition(ctx, &dg.g, hier.data(), (uint32_t) hier.size(), imbalance, seed,
+                                    config == "HM-ultra", dpart);
+            if (rc != HEIPA_OK) cli::die("heipa_hm_partition", rc, ctx);
+        }
         std::vector<int32_t> D(k * k);
         heipa_distance_matrix(hier.data(), dist.data(), (uint32_t) hier.size(), D.data());
         CLI_HIP(hipMalloc(&dD, 4ull * k * k));

@@ -45,3 +45,17 @@ def test_gpu_heipromap_cli(tmp_path):
           "--arena-gb", "1"])
     part = np.loadtxt(out, dtype=np.uint32)
     assert np.array_equal(part, orc.hm_partition(g, [2, 4, 3], 0.03, 1, False))
+
+
+def test_gpu_heipromap_cli_im(tmp_path):
+    g = heipa.gen_rgg(20000, 10.0, 8, 10)
+    path = str(tmp_path / "rgg.graph")
+    heipa.write_metis(path, g)
+    out = str(tmp_path / "map.txt")
+    stdout = _run([os.path.join(BIN, "GPU-HeiProMap"), "--graph", path, "--hierarchy", "4:8:6", "--distance",
+                   "1:10:100", "--imbalance", "0.03", "--config", "IM", "--seed", "1", "--mapping", out,
+                   "--verbose-level", "1", "--arena-gb", "1"])
+    part = np.loadtxt(out, dtype=np.uint32)
+    ref, J, _, _ = orc.im_partition(g, [4, 8, 6], [1, 10, 100], 0.03, 1)
+    assert np.array_equal(part, ref)
+    assert f"Final comm-cost   : {J}" in stdout
