@@ -56,8 +56,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device (the known-size kernel that scales HBM counters)")
     ap.add_argument("--slots", type=int, default=8, help="batched/served: pool workers")
-    ap.add_argument("--block-jet-n", type=int, default=1024,
-                    help="batched/served: largest level run by the single-workgroup Jet")
+    ap.add_argument("--block-jet-n", type=int, default=None,
+                    help="largest level run by the single-workgroup Jet (default: the library's)")
     ap.add_argument("--batch-count", type=int, default=4096, help="batched: number of graphs")
     ap.add_argument("--plain-sample", type=int, default=256, help="batched: graphs timed through the plain path")
     ap.add_argument("--requests", type=int, default=240, help="served: number of requests")
@@ -84,6 +84,8 @@ def main():
     g = make_graph(w)
     m_und = g.m // 2
     ctx = heipa.Context(0, 16 << 30)
+    if args.block_jet_n is not None:
+        ctx.set_option(heipa.OPT_BLOCK_JET, args.block_jet_n)
     dg = heipa.DeviceGraph(g, "cuda:0")
     part = torch.empty(g.n, dtype=torch.int32, device="cuda:0")
     k, ultra = w["k"], w["ultra"]
@@ -312,7 +314,8 @@ def run_pool_workload(args):
     plain_rate = sum(edges[i] for i in sample) / plain_s
 
     pool = heipa.Pool(args.slots, arena)
-    pool.set_option(heipa.OPT_BLOCK_JET, args.block_jet_n)
+    if args.block_jet_n is not None:
+        pool.set_option(heipa.OPT_BLOCK_JET, args.block_jet_n)
     warm = [heipa.Pool.job(dgs[i], imbalance=0.03, seed=1, **reqs[i][1]) for i in range(min(2 * args.slots, len(reqs)))]
     pool.run(warm)
     del warm

@@ -155,6 +155,7 @@ struct Ctx {
     u32 resolve_cap = ~0u;   // HEIPA_OPT_RESOLVE_CAP
     u32 block_jet_max_n = 1024;  // HEIPA_OPT_BLOCK_JET
     bool use_graphs = true;  // HEIPA_OPT_GRAPHS
+    u32 bucket_copies = 0;   // HEIPA_OPT_BUCKET_COPIES (0: automatic)
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;

@@ -154,7 +154,8 @@ struct JetDev {
     // rebalancing
     u32 sections, width, t;
     u32 *gid;           // minibucket per vertex
-    s64 *btot;          // t + 1 minibucket totals, zero between iterations
+    s64 *btot;          // nshard copies of the t + 1 minibucket totals, zero between iterations
+    u32 nshard;         // workgroup w adds into copy w % nshard (spreads same-address atomics)
     s64 *Q, *R;         // t: exclusive / inclusive prefix of btot inside each block's range
     s64 *Tseg;          // k: total of each block's range
     u64 *rlist;         // resolve list: minibucket << 32 | vertex
@@ -165,6 +166,14 @@ struct JetDev {
 };
 
 __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64) d.vw[u] : 1; }
+
+// a candidate's weight into minibucket g: rebalancing puts most candidates of
+// an overloaded block into a few dozen minibuckets, so each workgroup adds into
+// its own copy of the totals (the bucket kernel sums the copies)
+__device__ __forceinline__ void btot_add(const JetDev &d, u32 g, s64 w) {
+    const u64 copy = d.nshard > 1 ? blockIdx.x % d.nshard : 0;
+    atomicAdd((unsigned long long *) &d.btot[copy * ((u64) d.t + 1) + g], (unsigned long long) w);
+}
 __device__ __forceinline__ s64 ewt(const JetDev &d, u32 j) { return d.ew ? (s64) d.ew[j] : 1; }
 __device__ __forceinline__ u32 degree(const JetDev &d, u32 u) { return d.xadj[u + 1] - d.xadj[u]; }
 __device__ __forceinline__ bool live(const JetCtl *ctl) { return ctl->active && !ctl->stalled; }
@@ -475,7 +484,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
                 const u32 bt = gain_bucket(gain, vwt(d, u));
                 const u32 g = (MAX_BUCKETS * own_b + bt) * d.sections + (u % d.sections);
                 d.gid[u] = g;
-                atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) vwt(d, u));
+                btot_add(d, g, vwt(d, u));
                 listed = true;
             }
         }
@@ -501,7 +510,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
             const u32 bt = gain_bucket(gain, wu < lim ? wu : lim);
             const u32 g = (MAX_BUCKETS * own_b + bt) * d.sections + (u % d.sections) + 1;
             d.gid[u] = g;
-            atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) wu);
+            btot_add(d, g, wu);
             listed = true;
         }
     }
@@ -522,13 +531,13 @@ __device__ __forceinline__ bool reb_interior(const JetDev &d, bool strong, const
         if (dst == own_b) return false;
         const u32 g = (MAX_BUCKETS * own_b + gain_bucket(-own, wu)) * d.sections + (u % d.sections);
         d.gid[u] = g;
-        atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) wu);
+        btot_add(d, g, wu);
         return true;
     }
     const s64 lim = d.bw[own_b] - d.lmax;
     const u32 g = (MAX_BUCKETS * own_b + gain_bucket(-own, wu < lim ? wu : lim)) * d.sections + (u % d.sections) + 1;
     d.gid[u] = g;
-    atomicAdd((unsigned long long *) &d.btot[g], (unsigned long long) wu);
+    btot_add(d, g, wu);
     return true;
 }
 
@@ -659,7 +668,9 @@ __global__ __launch_bounds__(JB) void bucket_kernel(JetDev d) {
         s64 sum = 0;
 #pragma unroll
         for (int j = 0; j < BKT_ITEMS; ++j) {
-            v[j] = first + j < d.width ? d.btot[base + first + j] : 0;
+            v[j] = 0;
+            if (first + j < d.width)
+                for (u32 c = 0; c < d.nshard; ++c) v[j] += d.btot[(u64) c * (d.t + 1) + base + first + j];
             sum += v[j];
         }
         s64 tot;
@@ -670,14 +681,15 @@ __global__ __launch_bounds__(JB) void bucket_kernel(JetDev d) {
                 d.Q[base + first + j] = run;
                 run += v[j];
                 d.R[base + first + j] = run;
-                d.btot[base + first + j] = 0;
+                for (u32 c = 0; c < d.nshard; ++c) d.btot[(u64) c * (d.t + 1) + base + first + j] = 0;
             }
         }
         carry += tot;
     }
     if (threadIdx.x == 0) {
         d.Tseg[p] = carry;
-        if (p == d.k - 1) d.btot[(u64) d.k * d.width] = 0;  // strong +1 entry of the last block
+        if (p == d.k - 1)  // strong +1 entry of the last block
+            for (u32 c = 0; c < d.nshard; ++c) d.btot[(u64) c * (d.t + 1) + (u64) d.k * d.width] = 0;
     }
 }
 
@@ -2020,7 +2032,6 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         d.width = MAX_BUCKETS * s;
     }
     d.gid = bs.get<u32>(n);
-    d.btot = bs.get<s64>((u64) d.t + 1);
     d.Q = bs.get<s64>(d.t);
     d.R = bs.get<s64>(d.t);
     d.Tseg = bs.get<s64>(k);
@@ -2064,6 +2075,19 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         h.bm_lds = std::max(bm_sweep_lds_bytes(k, h.bm_groups), resolve_lds_bytes(BM_RCAP));
         h.block_mode = n <= c.block_jet_max_n && meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
     }
+    // copies of the minibucket totals: about one per 64 sweep workgroups, at
+    // most 32 and 8 MiB; the single-workgroup form keeps one
+    {
+        const u64 per_copy = 8ull * ((u64) d.t + 1);
+        const u64 wgs = (u64) div_up(n, 256);
+        u32 s = 1;
+        if (c.bucket_copies)
+            while (!h.block_mode && s * 2 <= c.bucket_copies) s *= 2;
+        else
+            while (!h.block_mode && s < 32 && (u64) s * 2 * 64 <= wgs && (u64) s * 2 * per_copy <= (8ull << 20)) s *= 2;
+        d.nshard = s;
+        d.btot = bs.get<s64>((u64) s * ((u64) d.t + 1));
+    }
     d.rcap = std::min<u32>(c.resolve_cap, h.big_resolve ? RESOLVE_CAP : RESOLVE_CAP_SMALL);
     // lane-group geometry: G lanes per vertex; the sweep keeps one LDS
     // k-counter array per group, so wide k shrinks the groups per workgroup and,
@@ -2105,7 +2129,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     fill<u8>(c, d.lock, n, (u8) 0);
     fill<u32>(c, d.mark, n, 0u);
     fill<u32>(c, h.over_flag, n, 0u);
-    fill<s64>(c, d.btot, (u64) d.t + 1, 0);
+    fill<s64>(c, d.btot, (u64) d.nshard * ((u64) d.t + 1), 0);
     block_weights_dev(c, g, d.work, k, d.bw);
     launch(c, "jet_init", 0.0, dim3(1), dim3(64), 0, ctl_init_kernel, d, rt, cut, maxw);
 

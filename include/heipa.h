@@ -227,6 +227,10 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
  * events, chosen by a hash of the launch count (default 1 = every launch); heipa_prof_get extrapolates
  * the timed launches' mean to all launches. */
 #define HEIPA_OPT_PROF_SAMPLE 4
+/* HEIPA_OPT_BUCKET_COPIES: copies of the rebalance minibucket totals that the multi-launch Jet
+ * spreads its atomic additions over (0 = automatic, default: about one per 64 sweep workgroups, at
+ * most 32; otherwise rounded down to a power of two, at most 32).  Results do not depend on it. */
+#define HEIPA_OPT_BUCKET_COPIES 5
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */

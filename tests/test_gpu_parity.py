@@ -153,6 +153,20 @@ def test_jet_global_resolve(ctx, graph, cap, block, ultra, level, skew, k):
         ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
 
 
+@pytest.mark.parametrize("copies", [2, 32])
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 1, 0.7, 7)])
+def test_jet_bucket_copies(ctx, graph, copies, ultra, level, skew, k):
+    """Rebalance minibucket totals spread over several copies (automatic only on
+    large levels) give the same iterations."""
+    ctx.set_option(heipa.OPT_BUCKET_COPIES, copies)
+    ctx.set_option(heipa.OPT_BLOCK_JET, 0)
+    try:
+        test_jet(ctx, graph, ultra, level, skew, k)
+    finally:
+        ctx.set_option(heipa.OPT_BUCKET_COPIES, 0)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
+
+
 @pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False), (3, False), (7, True)])
 def test_partition(ctx, graph, k, ultra):
     name, g = graph
