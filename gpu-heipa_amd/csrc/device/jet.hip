@@ -655,41 +655,64 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
 // differences inside one block's range).  Workgroup p scans block p's range
 // [p * width, (p + 1) * width) and clears the totals for the next iteration.
 constexpr int BKT_ITEMS = 8;
+constexpr u32 BKT_CHUNK = JB * BKT_ITEMS;
+constexpr u32 BKT_UNROLL = 8;  // copies loaded per round (loads in flight per thread)
 __global__ __launch_bounds__(JB) void bucket_kernel(JetDev d) {
     __shared__ s64 red[JB / WAVE];
+    __shared__ unsigned long long acc[BKT_CHUNK];
     JetCtl *ctl = d.ctl;
     if (!live(ctl) || ctl->mode == 0) return;
     const u32 p = blockIdx.x;
     const u64 base = (u64) p * d.width;
+    const u64 stride = (u64) d.t + 1;
     s64 carry = 0;
-    for (u32 c0 = 0; c0 < d.width; c0 += JB * BKT_ITEMS) {
-        const u32 first = c0 + threadIdx.x * BKT_ITEMS;
+    for (u32 c0 = 0; c0 < d.width; c0 += BKT_CHUNK) {
+        const u32 len = d.width - c0 < BKT_CHUNK ? d.width - c0 : BKT_CHUNK;
+        // sum the copies of this chunk into LDS and clear them; the loads of
+        // BKT_UNROLL copies are issued before any is used
+        for (u32 i = threadIdx.x; i < len; i += JB) acc[i] = 0;
+        __syncthreads();
+        for (u32 cb = 0; cb < d.nshard; cb += BKT_UNROLL) {
+            for (u32 i = threadIdx.x; i < len; i += JB) {
+                s64 x[BKT_UNROLL];
+#pragma unroll
+                for (u32 q = 0; q < BKT_UNROLL; ++q)
+                    x[q] = cb + q < d.nshard ? d.btot[(cb + q) * stride + base + c0 + i] : 0;
+#pragma unroll
+                for (u32 q = 0; q < BKT_UNROLL; ++q) {
+                    if (x[q]) {
+                        atomicAdd(&acc[i], (unsigned long long) x[q]);
+                        d.btot[(cb + q) * stride + base + c0 + i] = 0;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const u32 first = threadIdx.x * BKT_ITEMS;
         s64 v[BKT_ITEMS];
         s64 sum = 0;
 #pragma unroll
         for (int j = 0; j < BKT_ITEMS; ++j) {
-            v[j] = 0;
-            if (first + j < d.width)
-                for (u32 c = 0; c < d.nshard; ++c) v[j] += d.btot[(u64) c * (d.t + 1) + base + first + j];
+            v[j] = first + j < len ? (s64) acc[first + j] : 0;
             sum += v[j];
         }
         s64 tot;
         s64 run = carry + block_exclusive_scan<JB>(sum, red, tot);
 #pragma unroll
         for (int j = 0; j < BKT_ITEMS; ++j) {
-            if (first + j < d.width) {
-                d.Q[base + first + j] = run;
+            if (first + j < len) {
+                d.Q[base + c0 + first + j] = run;
                 run += v[j];
-                d.R[base + first + j] = run;
-                for (u32 c = 0; c < d.nshard; ++c) d.btot[(u64) c * (d.t + 1) + base + first + j] = 0;
+                d.R[base + c0 + first + j] = run;
             }
         }
         carry += tot;
+        __syncthreads();  // acc is refilled by the next chunk
     }
     if (threadIdx.x == 0) {
         d.Tseg[p] = carry;
         if (p == d.k - 1)  // strong +1 entry of the last block
-            for (u32 c = 0; c < d.nshard; ++c) d.btot[(u64) c * (d.t + 1) + (u64) d.k * d.width] = 0;
+            for (u32 c = 0; c < d.nshard; ++c) d.btot[c * stride + (u64) d.k * d.width] = 0;
     }
 }
 
