@@ -69,21 +69,44 @@ __device__ __forceinline__ u32 tier_of(u32 ub) {
     return 5;
 }
 
-__global__ void classify_kernel(u32 nc, const u32 *__restrict__ ub, u32 *__restrict__ tier_cnt,
-                                u32 *__restrict__ tier_lists, u32 stride) {
-    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < nc; i0 += (u64) gridDim.x * blockDim.x) {
+// Each workgroup takes a contiguous chunk of coarse vertices: it counts them
+// per tier in LDS, reserves each tier's space with one counter update, then
+// writes its entries (per-wave updates of the six counters serialised).
+__global__ __launch_bounds__(256) void classify_kernel(u32 nc, const u32 *__restrict__ ub, u32 *__restrict__ tier_cnt,
+                                                       u32 *__restrict__ tier_lists, u32 stride) {
+    __shared__ u32 s_cnt[6], s_base[6];
+    const u64 chunk = ((u64) nc + gridDim.x - 1) / gridDim.x;
+    const u64 c0 = (u64) blockIdx.x * chunk;
+    const u64 c1 = c0 + chunk < nc ? c0 + chunk : nc;
+    if (threadIdx.x < 6) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (u64 i0 = c0; i0 < c1; i0 += blockDim.x) {
         const u64 c = i0 + threadIdx.x;
-        const bool valid = c < nc;
-        const u32 t = valid ? tier_of(ub[c]) : 99u;
+        const u32 t = c < c1 ? tier_of(ub[c]) : 99u;
         for (u32 tt = 0; tt < 6; ++tt) {
-            const bool mine = valid && t == tt;
+            const u64 bal = __ballot(t == tt);
+            if (bal && lane_id() == 0) atomicAdd(&s_cnt[tt], (u32) __popcll(bal));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const u32 cnt = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = cnt ? atomicAdd(&tier_cnt[threadIdx.x], cnt) : 0u;
+        s_cnt[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (u64 i0 = c0; i0 < c1; i0 += blockDim.x) {
+        const u64 c = i0 + threadIdx.x;
+        const u32 t = c < c1 ? tier_of(ub[c]) : 99u;
+        for (u32 tt = 0; tt < 6; ++tt) {
+            const bool mine = t == tt;
             const u64 bal = __ballot(mine);
             if (!bal) continue;
             const u32 first = (u32) __builtin_ctzll(bal);
             u32 base = 0;
-            if (lane_id() == first) base = atomicAdd(&tier_cnt[tt], (u32) __popcll(bal));
+            if (lane_id() == first) base = atomicAdd(&s_cnt[tt], (u32) __popcll(bal));
             base = __shfl(base, (int) first, WAVE);
-            if (mine) tier_lists[(u64) tt * stride + base + (u32) __popcll(bal & lanemask_lt())] = (u32) c;
+            if (mine) tier_lists[(u64) tt * stride + s_base[tt] + base + (u32) __popcll(bal & lanemask_lt())] = (u32) c;
         }
     }
 }
@@ -390,7 +413,8 @@ CoarseGraph contract(Ctx &c, const DGraph &fine, const GraphMeta &meta, const u3
     s32 *tw = bs.get<s32>(mtot);
     const u32 stride = nc;
     u32 *tier_lists = bs.get<u32>((u64) 6 * stride);
-    launch(c, "contract_classify", 0.0, gc, blk, 0, classify_kernel, nc, (const u32 *) ub, scal, tier_lists, stride);
+    launch(c, "contract_classify", 0.0, dim3(grid_for(c, nc, 256, 2)), blk, 0, classify_kernel, nc, (const u32 *) ub,
+           scal, tier_lists, stride);
     u32 tc[6];
     readback(c, tc, scal, 6);
     launch_row_small<8>(c, fine, map, moff, mem, uboff, tier_lists + 0ull * stride, tc[0], tkey, tw, cnt, scal + 8);

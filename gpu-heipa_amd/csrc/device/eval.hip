@@ -14,14 +14,21 @@
 
 namespace heipa {
 
-__global__ void max_degree_kernel(const u32 *xadj, u32 n, u32 *out) {
+// one update per workgroup over a small grid (per-wave updates of one address serialise)
+__global__ __launch_bounds__(256) void max_degree_kernel(const u32 *xadj, u32 n, u32 *out) {
+    __shared__ u32 red[256 / WAVE];
     u32 mx = 0;
     for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64) gridDim.x * blockDim.x) {
         u32 d = xadj[u + 1] - xadj[u];
         mx = d > mx ? d : mx;
     }
     mx = wave_max(mx);
-    if (lane_id() == 0) atomicMax(out, mx);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (u32 w = 1; w < 256 / WAVE; ++w) mx = red[w] > mx ? red[w] : mx;
+        atomicMax(out, mx);
+    }
 }
 
 __global__ void hub_count_kernel(const u32 *xadj, u32 n, u32 hub_deg, u32 *count, u32 *list) {
@@ -44,7 +51,7 @@ GraphMeta make_meta(Ctx &c, const DGraph &g, bool back) {
         BackScope bs(c.arena);
         scr = bs.get<u32>(2);
         fill<u32>(c, scr, 2, 0u);
-        launch(c, "graph_meta", 4.0 * (g.n + 1), dim3(grid_for(c, g.n, 256)), dim3(256), 0, max_degree_kernel,
+        launch(c, "graph_meta", 4.0 * (g.n + 1), dim3(grid_for(c, g.n, 256, 2)), dim3(256), 0, max_degree_kernel,
                g.xadj, g.n, scr);
         launch(c, "graph_meta", 4.0 * (g.n + 1), dim3(grid_for(c, g.n, 256)), dim3(256), 0, hub_count_kernel,
                g.xadj, g.n, meta.hub_deg, scr + 1, (u32 *) nullptr);

@@ -155,6 +155,88 @@ __global__ __launch_bounds__(MB) void pick_kernel(DGraph g, u32 *__restrict__ vc
     }
 }
 
+// Repick and compaction (two_hop_matching.h:337-371).  Phase A: one thread
+// per list entry tests whether the vertex is unmatched with a matched
+// candidate and lists it in LDS; phase B: G lanes per listed vertex pick its
+// best unmatched neighbour.  Every unmatched vertex that still has a candidate
+// is kept for the next round; kept vertices collect in LDS and each workgroup
+// reserves their place in out_list with one counter update per buffer fill
+// (same-address atomics from every wave serialised the former form).  Hubs
+// were repicked by pick_hub_kernel before this launch.
+constexpr u32 RP_KEEP = 2048;
+template<int G, bool UEW>
+__global__ __launch_bounds__(MB) void repick_kernel(DGraph g, u32 *__restrict__ vcmap, u32 *__restrict__ hn, u32 seed,
+                                                    const u32 *__restrict__ list, const u32 *__restrict__ len_ptr,
+                                                    u32 hub_deg, u32 *__restrict__ out_list, u32 *__restrict__ out_len) {
+    __shared__ u32 s_work[MB];
+    __shared__ u32 s_keep[RP_KEEP];
+    __shared__ u32 s_nwork, s_nkeep, s_base;
+    const u32 len = list ? *len_ptr : g.n;
+    const u32 lg = threadIdx.x % G, gi = threadIdx.x / G;
+    if (threadIdx.x == 0) s_nkeep = 0;
+    auto flush = [&]() {
+        __syncthreads();
+        const u32 cnt = s_nkeep;
+        if (threadIdx.x == 0 && cnt) s_base = atomicAdd(out_len, cnt);
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < cnt; i += MB) out_list[s_base + i] = s_keep[i];
+        __syncthreads();
+        if (threadIdx.x == 0) s_nkeep = 0;
+        __syncthreads();
+    };
+    for (u64 p0 = (u64) blockIdx.x * MB; p0 < len; p0 += (u64) gridDim.x * MB) {
+        if (threadIdx.x == 0) s_nwork = 0;
+        __syncthreads();
+        const u64 i = p0 + threadIdx.x;
+        u32 u = 0, h = SENT;
+        bool unmatched = false, need = false;
+        if (i < len) {
+            u = list ? list[i] : (u32) i;
+            unmatched = vcmap[u] == SENT;
+            h = hn[u];
+            if (unmatched && h != SENT) need = vcmap[h] != SENT && g.xadj[u + 1] - g.xadj[u] <= hub_deg;
+        }
+        u32 pos = 0;
+        {
+            const u64 bal = __ballot(need);
+            const u32 first = bal ? (u32) __builtin_ctzll(bal) : 0;
+            u32 base = 0;
+            if (bal && lane_id() == first) base = atomicAdd(&s_nwork, (u32) __popcll(bal));
+            base = __shfl(base, (int) first, WAVE);
+            pos = base + (u32) __popcll(bal & lanemask_lt());
+            if (need) s_work[pos] = u;
+        }
+        __syncthreads();
+        const u32 nw = s_nwork;
+        for (u32 r0 = 0; r0 < nw; r0 += MB / G) {
+            const u32 idx = r0 + gi;
+            const bool act = idx < nw;
+            const u32 w = act ? s_work[idx] : 0;
+            Cand c{0, 0, SENT};
+            if (act) c = pick_partial<false, UEW>(g, vcmap, g.xadj[w], g.xadj[w + 1], lg, G, xorshift32(w ^ seed));
+            c = cand_reduce<G>(c);
+            if (act && lg == 0) {  // only this group reads or writes entry idx
+                hn[w] = c.v;
+                s_work[idx] = c.v;  // the new candidate, at the vertex's list position
+            }
+        }
+        __syncthreads();
+        if (need) h = s_work[pos];
+        const bool keep = unmatched && h != SENT;
+        if (s_nkeep + MB > RP_KEEP) flush();  // uniform: s_nkeep was last written before a barrier
+        {
+            const u64 bal = __ballot(keep);
+            const u32 first = bal ? (u32) __builtin_ctzll(bal) : 0;
+            u32 base = 0;
+            if (bal && lane_id() == first) base = atomicAdd(&s_nkeep, (u32) __popcll(bal));
+            base = __shfl(base, (int) first, WAVE);
+            if (keep) s_keep[base + (u32) __popcll(bal & lanemask_lt())] = u;
+        }
+        __syncthreads();
+    }
+    flush();
+}
+
 // hubs: one workgroup per hub
 template<bool INITIAL, bool UEW>
 __global__ __launch_bounds__(MB) void pick_hub_kernel(DGraph g, u32 *__restrict__ vcmap, u32 *__restrict__ hn,
@@ -210,12 +292,15 @@ __global__ void resolve_kernel(u32 n, u32 *__restrict__ vcmap, u32 *__restrict__
     }
 }
 
-__global__ void count_unmapped_kernel(const u32 *vcmap, u32 n, u32 *out) {
+// one counter update per workgroup (the grid is small): per-wave updates of
+// one address serialise
+__global__ __launch_bounds__(256) void count_unmapped_kernel(const u32 *vcmap, u32 n, u32 *out) {
+    __shared__ u64 red[256 / WAVE];
     u32 c = 0;
     for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64) gridDim.x * blockDim.x)
         c += vcmap[u] == SENT;
-    c = wave_sum(c);
-    if (lane_id() == 0 && c) atomicAdd(out, c);
+    c = (u32) block_sum<256>((u64) c, red);
+    if (threadIdx.x == 0 && c) atomicAdd(out, c);
 }
 
 // ---- keyed pairing (leaf / twin / relative)
@@ -367,17 +452,25 @@ template<int G>
 void launch_pick(Ctx &c, const DGraph &g, const GraphMeta &meta, bool initial, u32 *vcmap, u32 *hn, u32 seed,
                  const u32 *list, const u32 *len_ptr, u32 len_bound, u32 *out_list, u32 *out_len) {
     const bool uew = g.ew == nullptr;
-    const dim3 grid(grid_for(c, len_bound, MB / G, 8)), block(MB);
+    const dim3 block(MB);
     const double bytes = 0.0;
-#define PK(I, U) launch(c, I ? "match_initial_pick" : "match_repick", bytes, grid, block, 0, pick_kernel<G, I, U>, g, vcmap, hn, seed, list, len_ptr, meta.hub_deg, out_list, out_len)
     if (initial) {
-        if (uew) PK(true, true);
-        else PK(true, false);
+        const dim3 grid(grid_for(c, len_bound, MB / G, 8));
+        if (uew)
+            launch(c, "match_initial_pick", bytes, grid, block, 0, pick_kernel<G, true, true>, g, vcmap, hn, seed,
+                   list, len_ptr, meta.hub_deg, out_list, out_len);
+        else
+            launch(c, "match_initial_pick", bytes, grid, block, 0, pick_kernel<G, true, false>, g, vcmap, hn, seed,
+                   list, len_ptr, meta.hub_deg, out_list, out_len);
     } else {
-        if (uew) PK(false, true);
-        else PK(false, false);
+        const dim3 grid(grid_for(c, len_bound, MB, 2));
+        if (uew)
+            launch(c, "match_repick", bytes, grid, block, 0, repick_kernel<G, true>, g, vcmap, hn, seed, list, len_ptr,
+                   meta.hub_deg, out_list, out_len);
+        else
+            launch(c, "match_repick", bytes, grid, block, 0, repick_kernel<G, false>, g, vcmap, hn, seed, list,
+                   len_ptr, meta.hub_deg, out_list, out_len);
     }
-#undef PK
 }
 
 template<int G>
@@ -397,7 +490,7 @@ void launch_relative_keys(Ctx &c, const DGraph &g, const u32 *list, const u32 *l
 
 u32 count_unmapped(Ctx &c, const u32 *vcmap, u32 n, u32 *scratch) {
     fill<u32>(c, scratch, 1, 0u);
-    launch(c, "match_count_unmapped", 4.0 * n, dim3(grid_for(c, n, 256)), dim3(256), 0, count_unmapped_kernel, vcmap,
+    launch(c, "match_count_unmapped", 4.0 * n, dim3(grid_for(c, n, 256, 2)), dim3(256), 0, count_unmapped_kernel, vcmap,
            n, scratch);
     u32 h;
     readback(c, &h, scratch, 1);
