@@ -13,7 +13,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <queue>
+#include <system_error>
+#include <thread>
 #include <tuple>
 
 namespace heipa {
@@ -204,31 +207,61 @@ void fm2(const Sub &g, double max0, double max1, std::vector<u8> &side, int max_
     }
 }
 
+// Eight tries (graph growing from a random start, then FM), the best kept.
+// Every try draws n + 1 numbers from the splitmix64 stream, whose state
+// advances by a fixed constant per draw, so try t starts at
+// rng + t (n + 1) * constant and the tries of a large subgraph run on host
+// threads with the same result as one after the other.
 BisectResult bisect(const Sub &g, double tw0, double eps, u64 &rng) {
+    constexpr u64 STEP = 0x9e3779b97f4a7c15ull;  // splitmix64's state increment
     const double tw1 = (double) g.total - tw0;
     const double max0 = tw0 * (1.0 + eps), max1 = tw1 * (1.0 + eps);
-    BisectResult best;
-    bool have = false;
     const int tries = g.n < 8 ? 1 : 8;
-    std::vector<u64> prio(g.n);
-    std::vector<u8> side;
-    for (int t = 0; t < tries; ++t) {
-        for (u32 u = 0; u < g.n; ++u) prio[u] = splitmix64(rng);
-        u32 start = (u32) (splitmix64(rng) % g.n);
+    const u64 draws = (u64) g.n + 1;
+    const u64 rng0 = rng;
+    rng += (u64) tries * draws * STEP;
+    std::vector<BisectResult> res(tries);
+    auto one = [&](int t) {
+        u64 st = rng0 + (u64) t * draws * STEP;
+        std::vector<u64> prio(g.n);
+        for (u32 u = 0; u < g.n; ++u) prio[u] = splitmix64(st);
+        const u32 start = (u32) (splitmix64(st) % g.n);
+        std::vector<u8> side;
         grow(g, start, tw0, max0, prio, side);
         fm2(g, max0, max1, side, 8);
         s64 w0 = 0;
         for (u32 u = 0; u < g.n; ++u) if (side[u] == 0) w0 += g.vw[u];
-        s64 cut = compute_cut(g, side);
-        s64 ov = overload(w0, g.total - w0, max0, max1);
-        if (!have || better(ov, cut, best.over, best.cut)) {
-            best.side = side;
-            best.cut = cut;
-            best.over = ov;
-            have = true;
+        res[t].cut = compute_cut(g, side);
+        res[t].over = overload(w0, g.total - w0, max0, max1);
+        res[t].side = std::move(side);
+    };
+    std::vector<std::exception_ptr> errors(tries);
+    auto guarded = [&](int t) {
+        try {
+            one(t);
+        } catch (...) {
+            errors[t] = std::current_exception();
         }
+    };
+    std::vector<std::thread> workers;
+    for (int t = 1; t < tries; ++t) {
+        if (g.n >= 512) {
+            try {
+                workers.emplace_back(guarded, t);
+                continue;
+            } catch (const std::system_error &) {
+            }
+        }
+        guarded(t);  // small subgraph, or no thread available: this try runs here
     }
-    return best;
+    guarded(0);
+    for (auto &w: workers) w.join();
+    for (auto &e: errors)
+        if (e) std::rethrow_exception(e);
+    int best = 0;
+    for (int t = 1; t < tries; ++t)
+        if (better(res[t].over, res[t].cut, res[best].over, res[best].cut)) best = t;
+    return std::move(res[best]);
 }
 
 Sub extract(const Sub &g, const std::vector<u8> &side, u8 s) {

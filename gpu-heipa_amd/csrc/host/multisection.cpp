@@ -8,11 +8,17 @@
 // blocks, and so on; a leaf's block b is written as
 // offset(identifier path) + b.  Each split runs under the adaptive imbalance
 // that leaves the remaining levels the same slack.  Empty blocks are skipped:
-// they have no vertex to place.
+// they have no vertex to place.  The subtrees of the first split run on
+// host threads of their own: they share nothing but disjoint entries of the
+// output, and every split is deterministic, so the result does not depend on
+// the order in which they finish.
 #include "host_graph.hpp"
 
 #include <cmath>
+#include <exception>
 #include <stdexcept>
+#include <system_error>
+#include <thread>
 
 namespace heipa {
 
@@ -86,14 +92,40 @@ void split(u32 n, const u32 *xadj, const u32 *adj, const s32 *ew, const s32 *vw,
     }
     local.clear();
     local.shrink_to_fit();
-    for (u32 b = 0; b < k; ++b) {
+    auto child = [&](u32 b, std::vector<u32> &id) {
         Sub &s = subs[b];
-        if (s.n == 0) continue;
-        ident.push_back(b);
-        split(s.n, s.xadj.data(), s.adj.data(), s.ew.data(), s.vw.data(), s.gw, s.n_to_o.data(), level - 1, ident, p);
-        ident.pop_back();
+        id.push_back(b);
+        split(s.n, s.xadj.data(), s.adj.data(), s.ew.data(), s.vw.data(), s.gw, s.n_to_o.data(), level - 1, id, p);
+        id.pop_back();
         s = Sub{};
+    };
+    if (!ident.empty() || k < 2) {
+        for (u32 b = 0; b < k; ++b)
+            if (subs[b].n) child(b, ident);
+        return;
     }
+    std::vector<std::thread> workers;
+    std::vector<std::exception_ptr> errors(k);
+    for (u32 b = 0; b < k; ++b) {
+        if (subs[b].n == 0) continue;
+        auto run = [&, b]() {
+            try {
+                std::vector<u32> id;
+                id.reserve(p.hier.size());
+                child(b, id);
+            } catch (...) {
+                errors[b] = std::current_exception();
+            }
+        };
+        try {
+            workers.emplace_back(run);
+        } catch (const std::system_error &) {
+            run();  // no thread available: this subtree runs here
+        }
+    }
+    for (auto &t: workers) t.join();
+    for (auto &e: errors)
+        if (e) std::rethrow_exception(e);
 }
 
 } // namespace
