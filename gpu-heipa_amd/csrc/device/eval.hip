@@ -185,28 +185,51 @@ __global__ void validate_vertices_kernel(DGraph g, u32 *flags) {
             continue;
         }
         if (g.vw && g.vw[u] < 0) atomicOr(flags, 8u);
-        for (u32 j = b; j < e; ++j)
-            if (g.adj[j] == (u32) u) {
-                atomicOr(flags, 16u);
-                break;
-            }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (g.xadj[0] != 0 || g.xadj[g.n] != g.m)) atomicOr(flags, 1u);
 }
 
+// Edge checks, VAL_RUN consecutive entries per thread: the run's source vertex
+// comes from a binary search over xadj, then advances along the run (a hub's
+// self-loop check is spread over all threads instead of one).  A malformed
+// xadj (reported by the vertex kernel) only keeps the search in [0, n).
+constexpr u32 VAL_RUN = 32;
 __global__ void validate_edges_kernel(DGraph g, u32 *flags) {
-    for (u64 j = (u64) blockIdx.x * blockDim.x + threadIdx.x; j < g.m; j += (u64) gridDim.x * blockDim.x) {
-        if (g.adj[j] >= g.n) atomicOr(flags, 2u);
-        if (g.ew && g.ew[j] <= 0) atomicOr(flags, 4u);
+    const u64 runs = ((u64) g.m + VAL_RUN - 1) / VAL_RUN;
+    for (u64 r = (u64) blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += (u64) gridDim.x * blockDim.x) {
+        const u64 j0 = r * VAL_RUN, j1 = j0 + VAL_RUN < g.m ? j0 + VAL_RUN : g.m;
+        u32 lo = 0, hi = g.n;  // last u in [0, n) with xadj[u] <= j0
+        while (hi - lo > 1) {
+            const u32 mid = lo + (hi - lo) / 2;
+            if ((u64) g.xadj[mid] <= j0) lo = mid;
+            else hi = mid;
+        }
+        u32 u = lo;
+        u32 f = 0;
+        for (u64 j = j0; j < j1; ++j) {
+            while (u + 1 < g.n && (u64) g.xadj[u + 1] <= j) ++u;
+            const u32 v = g.adj[j];
+            if (v >= g.n) f |= 2u;
+            if (v == u) f |= 16u;
+            if (g.ew && g.ew[j] <= 0) f |= 4u;
+        }
+        if (f) atomicOr(flags, f);
     }
 }
 
-__global__ void max_part_kernel(const u32 *part, u32 n, u32 *out) {
+// one update per workgroup over a small grid
+__global__ __launch_bounds__(256) void max_part_kernel(const u32 *part, u32 n, u32 *out) {
+    __shared__ u32 red[256 / WAVE];
     u32 mx = 0;
     for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64) gridDim.x * blockDim.x)
         mx = part[u] > mx ? part[u] : mx;
     mx = wave_max(mx);
-    if (lane_id() == 0) atomicMax(out, mx);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (u32 w = 1; w < 256 / WAVE; ++w) mx = red[w] > mx ? red[w] : mx;
+        atomicMax(out, mx);
+    }
 }
 
 } // namespace
@@ -218,8 +241,8 @@ void validate_graph(Ctx &c, const DGraph &g) {
     launch(c, "validate", 4.0 * (g.n + 1), dim3(grid_for(c, g.n + 1, 256)), dim3(256), 0, validate_vertices_kernel, g,
            flags);
     if (g.m)
-        launch(c, "validate", 4.0 * g.m * (g.ew ? 2 : 1), dim3(grid_for(c, g.m, 256)), dim3(256), 0,
-               validate_edges_kernel, g, flags);
+        launch(c, "validate", 4.0 * g.m * (g.ew ? 2 : 1), dim3(grid_for(c, ((u64) g.m + VAL_RUN - 1) / VAL_RUN, 256)),
+               dim3(256), 0, validate_edges_kernel, g, flags);
     u32 f;
     readback(c, &f, flags, 1);
     if (f & 1u) throw Error(HEIPA_ERR_INVALID, "invalid graph: xadj is not a monotone offset array ending at m");
@@ -234,7 +257,7 @@ void validate_part(Ctx &c, const u32 *part, u32 n, u32 k) {
     BackScope bs(c.arena);
     u32 *mx = bs.get<u32>(1);
     fill<u32>(c, mx, 1, 0u);
-    launch(c, "validate", 4.0 * n, dim3(grid_for(c, n, 256)), dim3(256), 0, max_part_kernel, part, n, mx);
+    launch(c, "validate", 4.0 * n, dim3(grid_for(c, n, 256, 2)), dim3(256), 0, max_part_kernel, part, n, mx);
     u32 h;
     readback(c, &h, mx, 1);
     if (h >= k) throw Error(HEIPA_ERR_INVALID, "partition holds a block id >= k");

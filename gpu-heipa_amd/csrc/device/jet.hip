@@ -148,7 +148,7 @@ struct JetDev {
     s64 *bw;
     JetCtl *ctl;
     heipa_jet_trace *trace;  // ring of TRACE_RING records; null: no trace, no hashes
-    const double *ratios;
+    double ratios[MAX_RATIOS];  // filter ratios (in the kernel arguments: no memory round trip)
     u32 n_ratios, hub_deg;
     u32 rcap;  // resolve entries sorted in LDS (<= the kernel's capacity)
     // rebalancing
@@ -163,6 +163,7 @@ struct JetDev {
     s64 *evict_start;   // k + 1
     s64 *counter;       // strong-rebalance overflow position
     const s32 *D;       // k x k distances: mapping objective J (ProMap); null: edge cut
+    u32 count_bytes;    // hub kernels add their degree statistics (the byte accounting of the profiler)
 };
 
 __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64) d.vw[u] : 1; }
@@ -262,12 +263,32 @@ __device__ __forceinline__ GroupConn group_conn(char *smem, u32 k, u32 gidx) {
     return g;
 }
 
+// Adjacency loops take ADJ_UNROLL edges per lane per round and issue all
+// loads of a round before using any, so a vertex of degree D costs about
+// D / (stride * ADJ_UNROLL) rounds of dependent memory round trips instead of
+// D / stride (a hub's workgroup walks thousands of entries per thread group).
+constexpr u32 ADJ_UNROLL = 4;
+
 __device__ __forceinline__ void conn_accumulate(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, u32 G) {
-    const u32 b0 = d.xadj[u], e = d.xadj[u + 1];
-    for (u32 j = b0 + lane; j < e; j += G) {
-        const u32 b = d.work[d.adj[j]];
-        const unsigned long long old = atomicAdd(&gc.cnt[b], (unsigned long long) ewt(d, j));
-        if (old == 0) gc.touched[atomicAdd(gc.tcount, 1u)] = b;
+    const u64 b0 = d.xadj[u], e = d.xadj[u + 1];
+    for (u64 j0 = b0 + lane; j0 < e; j0 += (u64) G * ADJ_UNROLL) {
+        u32 v[ADJ_UNROLL], bl[ADJ_UNROLL];
+        s64 w[ADJ_UNROLL];
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            const u64 j = j0 + (u64) q * G;
+            v[q] = j < e ? d.adj[j] : 0u;
+            w[q] = j < e ? ewt(d, (u32) j) : 0;
+        }
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) bl[q] = j0 + (u64) q * G < e ? d.work[v[q]] : 0u;
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            if (j0 + (u64) q * G < e) {
+                const unsigned long long old = atomicAdd(&gc.cnt[bl[q]], (unsigned long long) w[q]);
+                if (old == 0) gc.touched[atomicAdd(gc.tcount, 1u)] = bl[q];
+            }
+        }
     }
 }
 
@@ -439,8 +460,7 @@ __device__ __forceinline__ void reb_prologue(const JetDev &d, bool strong, s64 m
 }
 
 // vertices considered for eviction (weak jet:641-646, strong jet:394-398)
-__device__ __forceinline__ bool reb_eligible(const JetDev &d, bool strong, const RebShared *rs, u32 u) {
-    const u32 ub = d.work[u];
+__device__ __forceinline__ bool reb_eligible(const JetDev &d, bool strong, const RebShared *rs, u32 u, u32 ub) {
     const s64 bwu = d.bw[ub];
     const s64 wu = vwt(d, u);
     if (!strong) return bwu > d.lmax && (double) wu < 1.5 * (double) (bwu - d.opt);
@@ -595,11 +615,12 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
                 if (save) d.part[v] = d.work[v];
                 if (mode == 0) {
                     const u32 dc = d.dcache[v];
+                    const u8 lk = d.lock[v];  // loaded with the cache entry: one round trip
                     if (dc == NULL_PART) {
                         heavy = degree(d, v) <= d.hub_deg;
                     } else {
                         d.dest[v] = dc;
-                        if (dc != NO_MOVE && d.lock[v] == 0) {
+                        if (dc != NO_MOVE && lk == 0) {
                             cand = true;
                             d.gain1[v] = d.gcache[v];
                         } else {
@@ -607,7 +628,7 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
                             d.lock[v] = 0;
                         }
                     }
-                } else if (degree(d, v) <= d.hub_deg && reb_eligible(d, strong, rs, v)) {
+                } else if (const u32 wv = d.work[v]; degree(d, v) <= d.hub_deg && reb_eligible(d, strong, rs, v, wv)) {
                     const s64 ic = d.icache[v];
                     if (ic >= 0) cand = reb_interior(d, strong, rs, v, ic);
                     else heavy = true;
@@ -738,7 +759,7 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     } else {
         mbw = max_block_weight(d, strong);
         reb_prologue(d, strong, mbw, rs);
-        if (!reb_eligible(d, strong, rs, u)) return;
+        if (!reb_eligible(d, strong, rs, u, d.work[u])) return;
     }
     for (u32 i = threadIdx.x; i < d.k; i += JB) gc.cnt[i] = 0;
     if (threadIdx.x == 0) *gc.tcount = 0;
@@ -749,8 +770,10 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
                                   : reb_eval<JB>(d, strong, mbw, rs, gc, u, threadIdx.x, red, red2, true);
     append(listed, u, d.cand_x, &ctl->st[ST_CAND]);
     if (threadIdx.x == 0) {
-        atomicAdd(&ctl->st[ST_MISS], 1ull);
-        atomicAdd(&ctl->st[ST_MISS_DEG], (unsigned long long) degree(d, u));
+        if (d.count_bytes) {  // one update per hub workgroup of two counters: only when profiling
+            atomicAdd(&ctl->st[ST_MISS], 1ull);
+            atomicAdd(&ctl->st[ST_MISS_DEG], (unsigned long long) degree(d, u));
+        }
     }
 }
 
@@ -760,27 +783,35 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
 __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 lane, u32 stride) {
     const s64 ug = d.gain1[u];
     const u32 o = d.work[u], nn = d.dest[u];
+    const s32 *Do = d.D ? d.D + (u64) o * d.k : nullptr, *Dn = d.D ? d.D + (u64) nn * d.k : nullptr;
     s64 change = 0;
-    if (d.D) {
-        // promap:297-314: the gain of u once v has moved minus its gain before
-        const s32 *Do = d.D + (u64) o * d.k, *Dn = d.D + (u64) nn * d.k;
-        for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
-            const u32 v = d.adj[j];
-            const s64 vg = d.gain1[v];
-            const bool first = vg > ug || (vg == ug && v < u);
-            if (!first) continue;
-            const u32 vn = d.dest[v], vo = d.work[v];
-            change += ewt(d, j) * (((s64) Do[vn] - (s64) Dn[vn]) - ((s64) Do[vo] - (s64) Dn[vo]));
+    const u64 e = d.xadj[u + 1];
+    for (u64 j0 = d.xadj[u] + lane; j0 < e; j0 += (u64) stride * ADJ_UNROLL) {
+        u32 v[ADJ_UNROLL], vn[ADJ_UNROLL], vo[ADJ_UNROLL];
+        s64 vg[ADJ_UNROLL], w[ADJ_UNROLL];
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            const u64 j = j0 + (u64) q * stride;
+            v[q] = j < e ? d.adj[j] : 0u;
+            w[q] = j < e ? ewt(d, (u32) j) : 0;
         }
-        return change;
-    }
-    for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
-        const u32 v = d.adj[j];
-        const s64 vg = d.gain1[v];
-        const bool first = vg > ug || (vg == ug && v < u);
-        if (!first) continue;
-        const u32 vn = d.dest[v], vo = d.work[v];
-        change += ewt(d, j) * ((s64) (vn == nn) - (s64) (vn == o) + (s64) (vo == o) - (s64) (vo == nn));
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            const bool ok = j0 + (u64) q * stride < e;
+            vg[q] = ok ? d.gain1[v[q]] : GAIN_MIN;
+            vn[q] = ok ? d.dest[v[q]] : 0u;
+            vo[q] = ok ? d.work[v[q]] : 0u;
+        }
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            if (j0 + (u64) q * stride >= e) continue;
+            const bool first = vg[q] > ug || (vg[q] == ug && v[q] < u);
+            if (!first) continue;
+            if (Do)  // promap:297-314: the gain of u once v has moved minus its gain before
+                change += w[q] * (((s64) Do[vn[q]] - (s64) Dn[vn[q]]) - ((s64) Do[vo[q]] - (s64) Dn[vo[q]]));
+            else
+                change += w[q] * ((s64) (vn[q] == nn) - (s64) (vn[q] == o) + (s64) (vo[q] == o) - (s64) (vo[q] == nn));
+        }
     }
     return change;
 }
@@ -813,18 +844,6 @@ __device__ __forceinline__ void afterburner_list(const JetDev &d, u32 stamp, con
             }
         }
         append_local(sel, u, mregion, s_cnt);
-    }
-}
-
-template<int G>
-__device__ __forceinline__ void afterburner_body(const JetDev &d, u32 stamp, u32 *mregion, u32 *s_cnt, u64 *s_red) {
-    u64 gh = 0, degs = 0;
-    afterburner_list<G>(d, stamp, d.cand + (u64) blockIdx.x * d.cap, d.cand_n[blockIdx.x], mregion, s_cnt, degs, gh);
-    degs = block_total(degs, s_red);
-    gh = block_total(gh, s_red);
-    if (threadIdx.x == 0) {
-        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_CAND_DEG] = degs;
-        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_GHASH] = gh;
     }
 }
 
@@ -862,9 +881,9 @@ __device__ __forceinline__ void classify_one(const JetDev &d, bool strong, u32 s
 }
 
 // this workgroup's candidate region, then its share of the extra list
-__device__ __forceinline__ void classify_body(const JetDev &d, bool strong, u32 stamp, u32 *mregion, u32 *s_cnt) {
+__device__ __forceinline__ void classify_body(const JetDev &d, bool strong, u32 stamp, u32 len, u32 *mregion,
+                                              u32 *s_cnt) {
     const u32 *region = d.cand + (u64) blockIdx.x * d.cap;
-    const u32 len = d.cand_n[blockIdx.x];
     for (u32 i0 = 0; i0 < len; i0 += blockDim.x) {
         const u32 i = i0 + threadIdx.x;
         classify_one(d, strong, stamp, i < len, i < len ? region[i] : 0u, mregion, s_cnt);
@@ -881,20 +900,146 @@ __device__ __forceinline__ void classify_body(const JetDev &d, bool strong, u32 
     }
 }
 
-template<int G>
+// ---------------------------------------------------------------- edge-balanced lists
+// A workgroup stages up to FLAT_CHUNK listed vertices in LDS with the
+// exclusive prefix of their degrees (hubs count 0: their own kernels handle
+// them) and walks the concatenated adjacencies with all its threads,
+// ADJ_UNROLL edges per thread per round; the vertex of edge x comes from a
+// binary search over the prefix.  Every thread gets the same number of edges
+// whatever the degree mix: with one vertex per lane group a workgroup waited
+// on its highest-degree vertex, and held few vertices in flight when the
+// lists are long (rebalancing on skewed graphs moves a large share of all
+// vertices per iteration).
+constexpr u32 FLAT_CHUNK = JB;
+struct FlatLds {
+    u32 u[FLAT_CHUNK];
+    u32 start[FLAT_CHUNK];  // xadj[u]
+    u32 off[FLAT_CHUNK + 1];
+    u32 a[FLAT_CHUNK], b[FLAT_CHUNK];
+    s64 g[FLAT_CHUNK];
+    unsigned long long acc[FLAT_CHUNK];
+    u32 red[JB / WAVE];
+};
+
+// last i in [0, cnt) with off[i] <= x (off non-decreasing, off[0] = 0)
+__device__ __forceinline__ u32 flat_find(const u32 *off, u32 cnt, u32 x) {
+    u32 lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// degrees of the staged vertices -> L.off; returns the chunk's edge count
+__device__ __forceinline__ u32 flat_prefix(FlatLds &L, u32 cnt, u32 deg) {
+    u32 tot;
+    const u32 ex = block_exclusive_scan<JB>(threadIdx.x < cnt ? deg : 0u, L.red, tot);
+    if (threadIdx.x < cnt) L.off[threadIdx.x] = ex;
+    if (threadIdx.x == 0) L.off[cnt] = tot;
+    __syncthreads();
+    return tot;
+}
+
+// afterburner (promap:284-322 / jet:269-328) over `len` candidates of
+// `region`, edge-balanced; selected vertices go to `mregion`
+__device__ void afterburner_flat(const JetDev &d, u32 stamp, const u32 *region, u32 len, u32 *mregion, u32 *s_cnt,
+                                 FlatLds &L, u64 &degs, u64 &gh) {
+    const bool do_hash = d.trace != nullptr;
+    for (u32 c0 = 0; c0 < len; c0 += FLAT_CHUNK) {
+        const u32 cnt = len - c0 < FLAT_CHUNK ? len - c0 : FLAT_CHUNK;
+        const u32 t = threadIdx.x;
+        u32 u = 0, deg = 0;
+        bool hub = true;
+        if (t < cnt) {
+            u = region[c0 + t];
+            const u32 b = d.xadj[u], e = d.xadj[u + 1];
+            hub = e - b > d.hub_deg;
+            deg = hub ? 0u : e - b;
+            L.u[t] = u;
+            L.start[t] = b;
+            L.g[t] = d.gain1[u];
+            L.a[t] = d.work[u];
+            L.b[t] = d.dest[u];
+            L.acc[t] = 0;
+        }
+        const u32 E = flat_prefix(L, cnt, deg);
+        for (u32 x0 = t; x0 < E; x0 += JB * ADJ_UNROLL) {
+            u32 iv[ADJ_UNROLL], v[ADJ_UNROLL], vn[ADJ_UNROLL], vo[ADJ_UNROLL];
+            s64 vg[ADJ_UNROLL], w[ADJ_UNROLL];
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+                const u32 x = x0 + q * JB;
+                const bool ok = x < E;
+                iv[q] = ok ? flat_find(L.off, cnt, x) : 0u;
+                const u32 j = L.start[iv[q]] + (x - L.off[iv[q]]);
+                v[q] = ok ? d.adj[j] : 0u;
+                w[q] = ok ? ewt(d, j) : 0;
+            }
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+                const bool ok = x0 + q * JB < E;
+                vg[q] = ok ? d.gain1[v[q]] : GAIN_MIN;
+                vn[q] = ok ? d.dest[v[q]] : 0u;
+                vo[q] = ok ? d.work[v[q]] : 0u;
+            }
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+                if (x0 + q * JB >= E) continue;
+                const u32 i = iv[q], cu = L.u[i];
+                const s64 ug = L.g[i];
+                if (!(vg[q] > ug || (vg[q] == ug && v[q] < cu))) continue;
+                const u32 o = L.a[i], nn = L.b[i];
+                s64 c;
+                if (d.D)  // promap:297-314
+                    c = w[q] * (((s64) d.D[(u64) o * d.k + vn[q]] - (s64) d.D[(u64) nn * d.k + vn[q]]) -
+                                ((s64) d.D[(u64) o * d.k + vo[q]] - (s64) d.D[(u64) nn * d.k + vo[q]]));
+                else
+                    c = w[q] * ((s64) (vn[q] == nn) - (s64) (vn[q] == o) + (s64) (vo[q] == o) - (s64) (vo[q] == nn));
+                if (c) atomicAdd(&L.acc[i], (unsigned long long) c);
+            }
+        }
+        __syncthreads();
+        bool sel = false;
+        if (t < cnt && !hub) {
+            degs += deg;
+            const s64 ug = L.g[t];
+            if (do_hash) gh += mix64d(mix64d(u) + (u64) ug);
+            if (ug + (s64) L.acc[t] >= 0) {
+                d.lock[u] = 1;
+                d.mark[u] = stamp;
+                d.oldp[u] = L.a[t];
+                sel = true;
+            }
+        }
+        append_local(sel, u, mregion, s_cnt);
+        __syncthreads();  // the next chunk restages the LDS
+    }
+}
+
 __global__ __launch_bounds__(JB) void post_kernel(JetDev d) {
     __shared__ u32 s_cnt;
     __shared__ u64 s_red[JB / WAVE];
+    __shared__ FlatLds L;
     JetCtl *ctl = d.ctl;
+    const u32 len = d.cand_n[blockIdx.x];  // issued with the control block: one round trip
     if (!live(ctl)) return;
     const u32 mode = ctl->mode, stamp = ctl->stamp;
     u32 *mregion = d.moves + (u64) blockIdx.x * d.cap_mv;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     if (mode == 0) {
-        afterburner_body<G>(d, stamp, mregion, &s_cnt, s_red);
+        u64 gh = 0, degs = 0;
+        afterburner_flat(d, stamp, d.cand + (u64) blockIdx.x * d.cap, len, mregion, &s_cnt, L, degs, gh);
+        degs = block_total(degs, s_red);
+        gh = block_total(gh, s_red);
+        if (threadIdx.x == 0) {
+            d.pstat[(u64) blockIdx.x * PT_COUNT + PT_CAND_DEG] = degs;
+            d.pstat[(u64) blockIdx.x * PT_COUNT + PT_GHASH] = gh;
+        }
     } else {
-        classify_body(d, mode == 2, stamp, mregion, &s_cnt);
+        classify_body(d, mode == 2, stamp, len, mregion, &s_cnt);
         if (threadIdx.x == 0) {
             d.pstat[(u64) blockIdx.x * PT_COUNT + PT_CAND_DEG] = 0;
             d.pstat[(u64) blockIdx.x * PT_COUNT + PT_GHASH] = 0;
@@ -916,7 +1061,7 @@ __global__ __launch_bounds__(JB) void hub_post_kernel(JetDev d, const u32 *hubs)
     ch = group_sum<JB>(ch, red);
     bool sel = false;
     if (threadIdx.x == 0) {
-        atomicAdd(&ctl->st[ST_CAND_DEG], (unsigned long long) degree(d, u));
+        if (d.count_bytes) atomicAdd(&ctl->st[ST_CAND_DEG], (unsigned long long) degree(d, u));
         if (d.trace) atomicAdd(&ctl->st[ST_GHASH], (unsigned long long) mix64d(mix64d(u) + (u64) ug));
         if (ug + ch >= 0) {
             d.lock[u] = 1;
@@ -1192,50 +1337,71 @@ __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
 // exact cut change, cache invalidation (u and N(u)), partition and block
 // weight update.  Non-moved neighbours are never written and moved
 // neighbours' old block comes from oldp, so concurrent writes to work[] are safe.
-__device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 stride, u32 stamp) {
+// the change of the objective over edges [b, e) of moved vertex u (lanes
+// `lane`, `lane + stride`, ...), invalidating the caches of the neighbours
+__device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u64 e, u32 lane, u32 stride,
+                                                u32 stamp) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
+    const s32 *Dn = d.D ? d.D + (u64) nu * d.k : nullptr, *Do = d.D ? d.D + (u64) ou * d.k : nullptr;
     s64 delta = 0;
-    if (d.D) {
-        // comm cost change (promap:1174-1207): moves that keep their block
-        // contribute nothing; an edge to a vertex that did not change block
-        // counts twice (its other direction is not visited)
-        const s32 *Dn = d.D + (u64) nu * d.k, *Do = d.D + (u64) ou * d.k;
-        for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
-            const u32 v = d.adj[j];
-            const bool mv = d.mark[v] == stamp;
-            const u32 ov = mv ? d.oldp[v] : d.work[v];
-            const u32 nv = mv ? d.dest[v] : ov;
-            if (ou != nu) {
-                const s64 x = ewt(d, j) * ((s64) Dn[nv] - (s64) Do[ov]);
-                delta += nv != ov ? x : 2 * x;
-            }
-            d.dcache[v] = NULL_PART;
-            d.icache[v] = -1;
+    for (u64 j0 = b + lane; j0 < e; j0 += (u64) stride * ADJ_UNROLL) {
+        u32 v[ADJ_UNROLL], mk[ADJ_UNROLL], op[ADJ_UNROLL], ds[ADJ_UNROLL], wk[ADJ_UNROLL];
+        s64 w[ADJ_UNROLL];
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            const u64 j = j0 + (u64) q * stride;
+            v[q] = j < e ? d.adj[j] : 0u;
+            w[q] = j < e ? ewt(d, (u32) j) : 0;
         }
-        return delta;
-    }
-    for (u32 j = d.xadj[u] + lane; j < d.xadj[u + 1]; j += stride) {
-        const u32 v = d.adj[j];
-        const bool mv = d.mark[v] == stamp;
-        const u32 ov = mv ? d.oldp[v] : d.work[v];
-        const u32 nv = mv ? d.dest[v] : ov;
-        const s64 x = ewt(d, j) * ((s64) (nu != nv) - (s64) (ou != ov));
-        delta += mv ? x : 2 * x;
-        d.dcache[v] = NULL_PART;
-        d.icache[v] = -1;
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            const bool ok = j0 + (u64) q * stride < e;
+            mk[q] = ok ? d.mark[v[q]] : 0u;
+            op[q] = ok ? d.oldp[v[q]] : 0u;
+            ds[q] = ok ? d.dest[v[q]] : 0u;
+            wk[q] = ok ? d.work[v[q]] : 0u;
+        }
+#pragma unroll
+        for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+            if (j0 + (u64) q * stride >= e) continue;
+            const bool mv = mk[q] == stamp;
+            const u32 ov = mv ? op[q] : wk[q];
+            const u32 nv = mv ? ds[q] : ov;
+            if (Dn) {
+                // comm cost change (promap:1174-1207): moves that keep their
+                // block contribute nothing; an edge to a vertex that did not
+                // change block counts twice (its other direction is not visited)
+                if (ou != nu) {
+                    const s64 x = w[q] * ((s64) Dn[nv] - (s64) Do[ov]);
+                    delta += nv != ov ? x : 2 * x;
+                }
+            } else {
+                const s64 x = w[q] * ((s64) (nu != nv) - (s64) (ou != ov));
+                delta += mv ? x : 2 * x;
+            }
+            d.dcache[v[q]] = NULL_PART;
+            d.icache[v[q]] = -1;
+        }
     }
     return delta;
 }
 
-__device__ __forceinline__ void move_commit(const JetDev &d, u32 u) {
+__device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 stride, u32 stamp) {
+    return move_delta_range(d, u, d.xadj[u], d.xadj[u + 1], lane, stride, stamp);
+}
+
+// commits a move; the block weight changes go to `bw` (the global array, or a
+// workgroup's LDS copy that is added to it once per block at the end)
+__device__ __forceinline__ void move_commit(const JetDev &d, u32 u, s64 *bw) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
     const s64 w = vwt(d, u);
     d.dcache[u] = NULL_PART;
     d.icache[u] = -1;
     d.work[u] = nu;
-    atomicAdd((unsigned long long *) &d.bw[ou], (unsigned long long) (-w));
-    atomicAdd((unsigned long long *) &d.bw[nu], (unsigned long long) w);
+    atomicAdd((unsigned long long *) &bw[ou], (unsigned long long) (-w));
+    atomicAdd((unsigned long long *) &bw[nu], (unsigned long long) w);
 }
+__device__ __forceinline__ void move_commit(const JetDev &d, u32 u) { move_commit(d, u, d.bw); }
 
 // algorithmic bytes of one iteration's launches (w = 2 when edge weights are
 // stored, else 1; E(X) = summed degree of the vertex set X):
@@ -1347,8 +1513,8 @@ __device__ void map_control(const JetDev &d, JetCtl *ctl, const unsigned long lo
 // One step of jet_refine's loop (jet:960-1030) after an iteration: counters,
 // cut and block weight, best-partition tracking, ratio and mode of the next
 // iteration.  Runs on one thread of the control launch.
-__device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot) {
-    JetCtl cl = *d.ctl;  // one batch of loads; written back at the end
+__device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot, const JetCtl &in) {
+    JetCtl cl = in;  // the control block as loaded by the caller; written back at the end
     JetCtl *ctl = &cl;
     unsigned long long rep[ST_COUNT];
 #pragma unroll
@@ -1430,21 +1596,45 @@ __device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot) {
     *d.ctl = cl;
 }
 
-__global__ __launch_bounds__(JB) void apply_hub_kernel(JetDev d, const u32 *hubs) {
+// hubs: one workgroup per HUB_CHUNK edges of a moved hub (the chunk table
+// lists (hub, chunk) pairs); chunk 0 also commits the move
+constexpr u32 HUB_CHUNK = JB * ADJ_UNROLL * 4;
+__global__ __launch_bounds__(JB) void apply_hub_kernel(JetDev d, const u32 *hubs, const u32 *chunk_hub,
+                                                       const u32 *chunk_idx) {
     __shared__ s64 red[JB / WAVE];
     JetCtl *ctl = d.ctl;
+    const u32 h = chunk_hub[blockIdx.x], ci = chunk_idx[blockIdx.x];
     if (!live(ctl)) return;
     const u32 stamp = ctl->stamp;
-    const u32 u = hubs[blockIdx.x];
+    const u32 u = hubs[h];
     if (d.mark[u] != stamp) return;
-    s64 acc = move_delta(d, u, threadIdx.x, JB, stamp);
+    const u64 b = (u64) d.xadj[u] + (u64) ci * HUB_CHUNK, end = d.xadj[u + 1];
+    const u64 e = b + HUB_CHUNK < end ? b + HUB_CHUNK : end;
+    s64 acc = move_delta_range(d, u, b, e, threadIdx.x, JB, stamp);
     acc = block_sum<JB>(acc, red);
     if (threadIdx.x == 0) {
         if (acc) atomicAdd(&ctl->st[ST_DELTA], (unsigned long long) acc);
-        atomicAdd(&ctl->st[ST_MOVE_DEG], (unsigned long long) degree(d, u));
-        move_commit(d, u);
-        if (d.trace) atomicAdd(&ctl->st[ST_MHASH], (unsigned long long) mix64d(((u64) u << 32) | d.dest[u]));
+        if (ci == 0) {
+            if (d.count_bytes) atomicAdd(&ctl->st[ST_MOVE_DEG], (unsigned long long) degree(d, u));
+            move_commit(d, u);
+            if (d.trace) atomicAdd(&ctl->st[ST_MHASH], (unsigned long long) mix64d(((u64) u << 32) | d.dest[u]));
+        }
     }
+}
+
+__global__ void hub_chunk_count_kernel(const u32 *xadj, const u32 *hubs, u32 n_hubs, u32 *cnt) {
+    for (u64 h = (u64) blockIdx.x * blockDim.x + threadIdx.x; h < n_hubs; h += (u64) gridDim.x * blockDim.x) {
+        const u32 u = hubs[h];
+        cnt[h] = (u32) (((u64) xadj[u + 1] - xadj[u] + HUB_CHUNK - 1) / HUB_CHUNK);
+    }
+}
+
+__global__ void hub_chunk_fill_kernel(const u32 *cnt, const u32 *off, u32 n_hubs, u32 *chunk_hub, u32 *chunk_idx) {
+    for (u64 h = (u64) blockIdx.x * blockDim.x + threadIdx.x; h < n_hubs; h += (u64) gridDim.x * blockDim.x)
+        for (u32 c = 0; c < cnt[h]; ++c) {
+            chunk_hub[off[h] + c] = (u32) h;
+            chunk_idx[off[h] + c] = c;
+        }
 }
 
 // G lanes per move: this workgroup's move region, then its share of the
@@ -1463,28 +1653,102 @@ __device__ __forceinline__ void apply_one(const JetDev &d, u32 u, bool active, u
     }
 }
 
-template<int G>
+// apply (perform_moves, jet:753-924 / promap:1104-1216): this workgroup's
+// move region, then its contiguous share of the extra list, edge-balanced
+constexpr u32 BW_LDS = 1024;  // blocks whose weight changes an apply workgroup sums in LDS
 __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
     __shared__ u64 s_red[JB / WAVE];
+    __shared__ FlatLds L;
+    __shared__ s64 s_bw[BW_LDS];
     JetCtl *ctl = d.ctl;
+    const u32 *region = d.moves + (u64) blockIdx.x * d.cap_mv;
+    // issued with the control block (one round trip)
+    const u32 len = d.moves_n[blockIdx.x];
+    const u32 lx = (u32) ctl->st[ST_MOVES];
     if (!live(ctl)) return;
     const u32 stamp = ctl->stamp;
     const bool do_hash = d.trace != nullptr;
-    const u32 groups = blockDim.x / G;
-    const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
-    const u32 *region = d.moves + (u64) blockIdx.x * d.cap_mv;
-    const u32 len = d.moves_n[blockIdx.x];
+    const u32 xch = (lx + gridDim.x - 1) / gridDim.x;
+    const u32 x0 = blockIdx.x * xch < lx ? blockIdx.x * xch : lx;
+    const u32 x1 = x0 + xch < lx ? x0 + xch : lx;
+    const u32 total = len + (x1 - x0);
+    // block weight changes: summed per workgroup in LDS and added once per
+    // block (a rebalancing iteration can move millions of vertices between
+    // few blocks; per-move updates of those few addresses serialised)
+    const bool lds_bw = d.k <= BW_LDS;
+    if (lds_bw)
+        for (u32 b = threadIdx.x; b < d.k; b += JB) s_bw[b] = 0;
+    __syncthreads();
+    s64 *bwt = lds_bw ? s_bw : d.bw;
     s64 acc = 0;
     u64 mh = 0, degs = 0;
-    for (u32 i0 = 0; i0 < len; i0 += groups) {
-        const u32 i = i0 + gi;
-        apply_one<G>(d, i < len ? region[i] : 0u, i < len, lane, stamp, do_hash, acc, degs, mh);
+    for (u32 c0 = 0; c0 < total; c0 += FLAT_CHUNK) {
+        const u32 cnt = total - c0 < FLAT_CHUNK ? total - c0 : FLAT_CHUNK;
+        const u32 t = threadIdx.x;
+        u32 u = 0, deg = 0;
+        bool hub = true;
+        if (t < cnt) {
+            const u32 i = c0 + t;
+            u = i < len ? region[i] : d.moves_x[x0 + (i - len)];
+            const u32 b = d.xadj[u], e = d.xadj[u + 1];
+            hub = e - b > d.hub_deg;
+            deg = hub ? 0u : e - b;
+            L.u[t] = u;
+            L.start[t] = b;
+            L.a[t] = d.oldp[u];
+            L.b[t] = d.dest[u];
+        }
+        const u32 E = flat_prefix(L, cnt, deg);
+        for (u32 y0 = t; y0 < E; y0 += JB * ADJ_UNROLL) {
+            u32 iv[ADJ_UNROLL], v[ADJ_UNROLL], mk[ADJ_UNROLL], op[ADJ_UNROLL], ds[ADJ_UNROLL], wk[ADJ_UNROLL];
+            s64 w[ADJ_UNROLL];
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+                const u32 y = y0 + q * JB;
+                const bool ok = y < E;
+                iv[q] = ok ? flat_find(L.off, cnt, y) : 0u;
+                const u32 j = L.start[iv[q]] + (y - L.off[iv[q]]);
+                v[q] = ok ? d.adj[j] : 0u;
+                w[q] = ok ? ewt(d, j) : 0;
+            }
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+                const bool ok = y0 + q * JB < E;
+                mk[q] = ok ? d.mark[v[q]] : 0u;
+                op[q] = ok ? d.oldp[v[q]] : 0u;
+                ds[q] = ok ? d.dest[v[q]] : 0u;
+                wk[q] = ok ? d.work[v[q]] : 0u;
+            }
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) {
+                if (y0 + q * JB >= E) continue;
+                const u32 ou = L.a[iv[q]], nu = L.b[iv[q]];
+                const bool mv = mk[q] == stamp;
+                const u32 ov = mv ? op[q] : wk[q];
+                const u32 nv = mv ? ds[q] : ov;
+                if (d.D) {  // promap:1174-1207
+                    if (ou != nu) {
+                        const s64 x = w[q] * ((s64) d.D[(u64) nu * d.k + nv] - (s64) d.D[(u64) ou * d.k + ov]);
+                        acc += nv != ov ? x : 2 * x;
+                    }
+                } else {
+                    const s64 x = w[q] * ((s64) (nu != nv) - (s64) (ou != ov));
+                    acc += mv ? x : 2 * x;
+                }
+                d.dcache[v[q]] = NULL_PART;
+                d.icache[v[q]] = -1;
+            }
+        }
+        if (t < cnt && !hub) {
+            degs += deg;
+            move_commit(d, u, bwt);
+            if (do_hash) mh += mix64d(((u64) u << 32) | L.b[t]);
+        }
+        __syncthreads();  // the next chunk restages the LDS
     }
-    const u32 lx = (u32) ctl->st[ST_MOVES];
-    for (u64 i0 = (u64) blockIdx.x * groups; i0 < lx; i0 += (u64) gridDim.x * groups) {
-        const u64 i = i0 + gi;
-        apply_one<G>(d, i < lx ? d.moves_x[i] : 0u, i < lx, lane, stamp, do_hash, acc, degs, mh);
-    }
+    if (lds_bw)
+        for (u32 b = threadIdx.x; b < d.k; b += JB)
+            if (s_bw[b]) atomicAdd((unsigned long long *) &d.bw[b], (unsigned long long) s_bw[b]);
     const u64 a = block_total((u64) acc, s_red);
     degs = block_total(degs, s_red);
     mh = block_total(mh, s_red);
@@ -1499,8 +1763,13 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
 // partials (one barrier), then control_step
 __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
     constexpr u32 NV = PT_COUNT + 3;  // partials, two list lengths, max block weight
+    constexpr u32 CTL_WORDS = sizeof(JetCtl) / 8;
+    static_assert(sizeof(JetCtl) % 8 == 0 && CTL_WORDS <= JB, "control block layout");
     __shared__ u64 s_part[JB / WAVE][NV];
-    if (!live(d.ctl)) return;
+    __shared__ u64 s_ctl[CTL_WORDS];
+    // the control block, the partials and the block weights are independent
+    // loads: issue them together (one memory round trip)
+    if (threadIdx.x < CTL_WORDS) s_ctl[threadIdx.x] = ((const u64 *) d.ctl)[threadIdx.x];
     u64 t[NV];
 #pragma unroll
     for (u32 j = 0; j < NV; ++j) t[j] = 0;
@@ -1522,6 +1791,8 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        const JetCtl &cur = *(const JetCtl *) s_ctl;
+        if (!cur.active || cur.stalled) return;
         s64 m = INT64_MIN;
 #pragma unroll
         for (u32 j = 0; j < NV - 1; ++j) t[j] = 0;
@@ -1530,7 +1801,7 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
             for (u32 j = 0; j < NV - 1; ++j) t[j] += s_part[w][j];
             m = (s64) s_part[w][NV - 1] > m ? (s64) s_part[w][NV - 1] : m;
         }
-        control_step(d, m, t);
+        control_step(d, m, t, cur);
     }
 }
 
@@ -1622,7 +1893,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
                                 d.lock[v] = 0;
                             }
                         }
-                    } else if (reb_eligible(d, strong, rs, v)) {
+                    } else if (reb_eligible(d, strong, rs, v, d.work[v])) {
                         const s64 ic = d.icache[v];
                         if (ic >= 0) cand = reb_interior(d, strong, rs, v, ic);
                         else heavy = true;
@@ -1776,7 +2047,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
             t[PT_COUNT] = nc;
             t[PT_COUNT + 1] = nm;
             ctl->st[ST_RLIST] = L;
-            control_step(d, m, t);
+            control_step(d, m, t, *d.ctl);
         }
         __syncthreads();
     }
@@ -1809,9 +2080,6 @@ __global__ void ctl_init_kernel(JetDev d, RatioTable rt, s64 cut, s64 maxw) {
     ctl->iters = 0;
     ctl->n_trace = 0;
     ctl->stamp = 1;
-    double *ratios = (double *) d.ratios;
-#pragma unroll
-    for (u32 i = 0; i < MAX_RATIOS; ++i) ratios[i] = rt.r[i];
 }
 
 // ---------------------------------------------------------------- global resolve path
@@ -1929,6 +2197,8 @@ struct JetHost {
     size_t sweep_smem = 0, hub_smem = 0;
     s64 *H = nullptr, *roff = nullptr;
     u32 *over_flag = nullptr, *olist = nullptr, *olen = nullptr;
+    u32 n_chunks = 0;  // hub chunks of the apply step
+    u32 *chunk_hub = nullptr, *chunk_idx = nullptr;
 
     JetHost(Ctx &cc, const DGraph &gg, const GraphMeta &mm) : c(cc), g(gg), meta(mm) {}
 
@@ -1936,19 +2206,12 @@ struct JetHost {
     void launch_sweep() {
         launch(c, "jet_sweep", 0.0, dim3(d.nreg), dim3(sweep_threads), sweep_smem, sweep_kernel<GG>, d);
     }
-    template<int GG>
-    void launch_post() {
-        launch(c, "jet_post", 0.0, dim3(d.nreg), dim3(JB), 0, post_kernel<GG>, d);
-    }
-    template<int GG>
-    void launch_apply() {
-        launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_kernel<GG>, d);
-    }
 
     void apply() {
-        if (meta.n_hubs)
-            launch(c, "jet_apply", 0.0, dim3(meta.n_hubs), dim3(JB), 0, apply_hub_kernel, d, (const u32 *) meta.hubs);
-        HEIPA_DISPATCH_G(G, launch_apply);
+        if (n_chunks)
+            launch(c, "jet_apply", 0.0, dim3(n_chunks), dim3(JB), 0, apply_hub_kernel, d, (const u32 *) meta.hubs,
+                   (const u32 *) chunk_hub, (const u32 *) chunk_idx);
+        launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_kernel, d);
         launch(c, "jet_control", 0.0, dim3(1), dim3(JB), 0, control_kernel, d);
     }
 
@@ -1968,7 +2231,7 @@ struct JetHost {
             launch(c, "jet_post", 0.0, dim3(meta.n_hubs), dim3(JB), 0, hub_post_kernel, d, (const u32 *) meta.hubs);
         }
         launch(c, "jet_buckets", 0.0, dim3(d.k), dim3(JB), 0, bucket_kernel, d);
-        HEIPA_DISPATCH_G(G, launch_post);
+        launch(c, "jet_post", 0.0, dim3(d.nreg), dim3(JB), 0, post_kernel, d);
         if (big_resolve)
             launch(c, "jet_resolve", 0.0, dim3(1), dim3(RB), 0, resolve_kernel<RESOLVE_CAP>, d);
         else
@@ -2030,6 +2293,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.vw = g.vw;
     d.hub_deg = meta.hub_deg;
     d.D = D;
+    d.count_bytes = c.prof_on ? 1u : 0u;
     if (D) ultra = false;  // ProMap_jet_refine runs one filter ratio, which its LP does not read
     d.part = part;
     d.work = bs.get<u32>(n);
@@ -2076,7 +2340,6 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     char *ctl_mem = bs.get<char>(snap_bytes);
     d.ctl = (JetCtl *) ctl_mem;
     d.trace = tracing ? (heipa_jet_trace *) (ctl_mem + ctl_bytes) : nullptr;
-    d.ratios = bs.get<double>(MAX_RATIOS);
     RatioTable rt{};
     {
         const double ur[MAX_RATIOS] = {0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45,
@@ -2089,6 +2352,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
             rt.n = 1;
         }
         d.n_ratios = rt.n;
+        for (u32 i = 0; i < MAX_RATIOS; ++i) d.ratios[i] = rt.r[i];
     }
     h.big_resolve = c.max_lds >= 160 * 1024;
     // single-workgroup form for small levels without hubs
@@ -2096,7 +2360,10 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         const size_t cgb = conn_group_bytes(k);
         h.bm_groups = (u32) std::max<size_t>(1, std::min<size_t>(BM / meta.G, BM_CONN_LDS / cgb));
         h.bm_lds = std::max(bm_sweep_lds_bytes(k, h.bm_groups), resolve_lds_bytes(BM_RCAP));
-        h.block_mode = n <= c.block_jet_max_n && meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
+        // one workgroup walks every edge each iteration: dense small levels
+        // (coarse levels of skewed graphs) stay on the multi-launch form
+        h.block_mode = n <= c.block_jet_max_n && (u64) g.m <= 32ull * c.block_jet_max_n && meta.n_hubs == 0 &&
+                       h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
     }
     // copies of the minibucket totals: about one per 64 sweep workgroups, at
     // most 32 and 8 MiB; the single-workgroup form keeps one
@@ -2145,6 +2412,18 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         d.pstat = bs.get<unsigned long long>((u64) d.nreg * PT_COUNT);
         d.cand_x = bs.get<u32>(meta.n_hubs + 1);
         d.moves_x = bs.get<u32>(n);
+    }
+    if (meta.n_hubs) {  // (hub, chunk) table of the hub apply step
+        u32 *cnt = bs.get<u32>(meta.n_hubs), *off = bs.get<u32>(meta.n_hubs + 1);
+        const dim3 hg(grid_for(c, meta.n_hubs, 256));
+        launch(c, "jet_init", 0.0, hg, dim3(256), 0, hub_chunk_count_kernel, g.xadj, (const u32 *) meta.hubs,
+               meta.n_hubs, cnt);
+        exclusive_sum<u32>(c, "jet_init", cnt, off, meta.n_hubs);
+        readback(c, &h.n_chunks, off + meta.n_hubs, 1);
+        h.chunk_hub = bs.get<u32>(h.n_chunks);
+        h.chunk_idx = bs.get<u32>(h.n_chunks);
+        launch(c, "jet_init", 0.0, hg, dim3(256), 0, hub_chunk_fill_kernel, (const u32 *) cnt, (const u32 *) off,
+               meta.n_hubs, h.chunk_hub, h.chunk_idx);
     }
     copy_d2d(c, d.work, part, 4ull * n);
     fill<u32>(c, d.dcache, n, NULL_PART);

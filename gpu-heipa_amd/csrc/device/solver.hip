@@ -93,13 +93,43 @@ __global__ void child_weight_kernel(const s32 *c_vw, u32 cn, unsigned long long 
     if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long) s);
 }
 
-// per child vertex: number of neighbours inside the child (thread per vertex)
-__global__ void child_degree_kernel(DGraph g, const u32 *tmp, u32 id, const u32 *c_n_to_o_local, u32 cn, u32 *cdeg) {
-    for (u64 i = (u64) blockIdx.x * blockDim.x + threadIdx.x; i < cn; i += (u64) gridDim.x * blockDim.x) {
-        const u32 u = c_n_to_o_local[i];
-        u32 d = 0;
-        for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) d += tmp[g.adj[j]] == id;
-        cdeg[i] = d;
+// per child vertex: the number of its neighbours inside the child.  A thread
+// counts a vertex of degree up to CD_SMALL; larger ones are listed in LDS and
+// counted by a whole wave each (a hub's adjacency on one thread serialised).
+constexpr u32 CD_SMALL = 32;
+__global__ __launch_bounds__(256) void child_degree_kernel(DGraph g, const u32 *tmp, u32 id,
+                                                           const u32 *c_n_to_o_local, u32 cn, u32 *cdeg) {
+    __shared__ u32 s_big[256];
+    __shared__ u32 s_nbig;
+    for (u64 i0 = (u64) blockIdx.x * blockDim.x; i0 < cn; i0 += (u64) gridDim.x * blockDim.x) {
+        if (threadIdx.x == 0) s_nbig = 0;
+        __syncthreads();
+        const u64 i = i0 + threadIdx.x;
+        if (i < cn) {
+            const u32 u = c_n_to_o_local[i];
+            const u32 b = g.xadj[u], e = g.xadj[u + 1];
+            if (e - b <= CD_SMALL) {
+                u32 d = 0;
+#pragma unroll 8
+                for (u32 j = b; j < e; ++j) d += tmp[g.adj[j]] == id;
+                cdeg[i] = d;
+            } else {
+                s_big[atomicAdd(&s_nbig, 1u)] = (u32) i;
+            }
+        }
+        __syncthreads();
+        const u32 nb = s_nbig;
+        for (u32 q = threadIdx.x / WAVE; q < nb; q += blockDim.x / WAVE) {
+            const u32 ii = s_big[q];
+            const u32 u = c_n_to_o_local[ii];
+            const u32 e = g.xadj[u + 1];
+            u32 d = 0;
+#pragma unroll 8
+            for (u32 j = g.xadj[u] + lane_id(); j < e; j += WAVE) d += tmp[g.adj[j]] == id;
+            d = wave_sum(d);
+            if (lane_id() == 0) cdeg[ii] = d;
+        }
+        __syncthreads();
     }
 }
 
