@@ -175,46 +175,51 @@ __global__ __launch_bounds__(256) void block_weights_kernel(u32 n, const u32 *__
 }
 
 // input validation: xadj monotone within [0, m], neighbour ids < n, edge
-// weights > 0, vertex weights >= 0; flags bit 0 = bad offsets, 1 = bad id,
-// 2 = bad edge weight, 3 = bad vertex weight
-__global__ void validate_vertices_kernel(DGraph g, u32 *flags) {
-    for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < g.n; u += (u64) gridDim.x * blockDim.x) {
-        const u32 b = g.xadj[u], e = g.xadj[u + 1];
-        if (b > e || e > g.m) {
-            atomicOr(flags, 1u);
-            continue;
+// weights > 0, vertex weights >= 0, no self loops; flags bit 0 = bad offsets,
+// 1 = bad id, 2 = bad edge weight, 3 = bad vertex weight, 4 = self loop.
+// The self-loop scan runs here: a thread scans an adjacency of up to
+// VAL_SMALL entries, longer ones are listed in LDS and scanned by a whole wave
+// each (a hub's adjacency on one thread would serialise the launch).
+constexpr u32 VAL_SMALL = 32;
+__global__ __launch_bounds__(256) void validate_vertices_kernel(DGraph g, u32 *flags) {
+    __shared__ u32 s_big[256];
+    __shared__ u32 s_nbig;
+    for (u64 u0 = (u64) blockIdx.x * blockDim.x; u0 < g.n; u0 += (u64) gridDim.x * blockDim.x) {
+        if (threadIdx.x == 0) s_nbig = 0;
+        __syncthreads();
+        const u64 u = u0 + threadIdx.x;
+        u32 f = 0;
+        if (u < g.n) {
+            const u32 b = g.xadj[u], e = g.xadj[u + 1];
+            if (b > e || e > g.m) {
+                f |= 1u;
+            } else if (e - b <= VAL_SMALL) {
+                for (u32 j = b; j < e; ++j) f |= g.adj[j] == (u32) u ? 16u : 0u;
+            } else {
+                s_big[atomicAdd(&s_nbig, 1u)] = (u32) u;
+            }
+            if (g.vw && g.vw[u] < 0) f |= 8u;
         }
-        if (g.vw && g.vw[u] < 0) atomicOr(flags, 8u);
+        __syncthreads();
+        const u32 nb = s_nbig;
+        for (u32 q = threadIdx.x / WAVE; q < nb; q += blockDim.x / WAVE) {
+            const u32 v = s_big[q];
+            const u32 e = g.xadj[v + 1];
+            for (u32 j = g.xadj[v] + lane_id(); j < e; j += WAVE) f |= g.adj[j] == v ? 16u : 0u;
+        }
+        if (f) atomicOr(flags, f);
+        __syncthreads();
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (g.xadj[0] != 0 || g.xadj[g.n] != g.m)) atomicOr(flags, 1u);
 }
 
-// Edge checks, VAL_RUN consecutive entries per thread: the run's source vertex
-// comes from a binary search over xadj, then advances along the run (a hub's
-// self-loop check is spread over all threads instead of one).  A malformed
-// xadj (reported by the vertex kernel) only keeps the search in [0, n).
-constexpr u32 VAL_RUN = 32;
 __global__ void validate_edges_kernel(DGraph g, u32 *flags) {
-    const u64 runs = ((u64) g.m + VAL_RUN - 1) / VAL_RUN;
-    for (u64 r = (u64) blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += (u64) gridDim.x * blockDim.x) {
-        const u64 j0 = r * VAL_RUN, j1 = j0 + VAL_RUN < g.m ? j0 + VAL_RUN : g.m;
-        u32 lo = 0, hi = g.n;  // last u in [0, n) with xadj[u] <= j0
-        while (hi - lo > 1) {
-            const u32 mid = lo + (hi - lo) / 2;
-            if ((u64) g.xadj[mid] <= j0) lo = mid;
-            else hi = mid;
-        }
-        u32 u = lo;
-        u32 f = 0;
-        for (u64 j = j0; j < j1; ++j) {
-            while (u + 1 < g.n && (u64) g.xadj[u + 1] <= j) ++u;
-            const u32 v = g.adj[j];
-            if (v >= g.n) f |= 2u;
-            if (v == u) f |= 16u;
-            if (g.ew && g.ew[j] <= 0) f |= 4u;
-        }
-        if (f) atomicOr(flags, f);
+    u32 f = 0;
+    for (u64 j = (u64) blockIdx.x * blockDim.x + threadIdx.x; j < g.m; j += (u64) gridDim.x * blockDim.x) {
+        f |= g.adj[j] >= g.n ? 2u : 0u;
+        if (g.ew && g.ew[j] <= 0) f |= 4u;
     }
+    if (f) atomicOr(flags, f);
 }
 
 // one update per workgroup over a small grid
@@ -241,8 +246,8 @@ void validate_graph(Ctx &c, const DGraph &g) {
     launch(c, "validate", 4.0 * (g.n + 1), dim3(grid_for(c, g.n + 1, 256)), dim3(256), 0, validate_vertices_kernel, g,
            flags);
     if (g.m)
-        launch(c, "validate", 4.0 * g.m * (g.ew ? 2 : 1), dim3(grid_for(c, ((u64) g.m + VAL_RUN - 1) / VAL_RUN, 256)),
-               dim3(256), 0, validate_edges_kernel, g, flags);
+        launch(c, "validate", 4.0 * g.m * (g.ew ? 2 : 1), dim3(grid_for(c, g.m, 256)), dim3(256), 0,
+               validate_edges_kernel, g, flags);
     u32 f;
     readback(c, &f, flags, 1);
     if (f & 1u) throw Error(HEIPA_ERR_INVALID, "invalid graph: xadj is not a monotone offset array ending at m");

@@ -145,17 +145,18 @@ struct JetDev {
     u32 *cand_x, *moves_x;  // extra lists (lengths in st[ST_CAND], st[ST_MOVES])
     unsigned long long *pstat;  // PT_COUNT partial statistics per sweep workgroup
     u32 nreg, cap, cap_mv;
+    u32 count_bytes;  // hub kernels add their degree statistics (the byte accounting of the profiler)
     s64 *bw;
     JetCtl *ctl;
     heipa_jet_trace *trace;  // ring of TRACE_RING records; null: no trace, no hashes
-    double ratios[MAX_RATIOS];  // filter ratios (in the kernel arguments: no memory round trip)
+    const double *ratios;  // MAX_RATIOS filter ratios
     u32 n_ratios, hub_deg;
     u32 rcap;  // resolve entries sorted in LDS (<= the kernel's capacity)
     // rebalancing
     u32 sections, width, t;
+    u32 nshard;         // workgroup w adds into copy w % nshard (spreads same-address atomics)
     u32 *gid;           // minibucket per vertex
     s64 *btot;          // nshard copies of the t + 1 minibucket totals, zero between iterations
-    u32 nshard;         // workgroup w adds into copy w % nshard (spreads same-address atomics)
     s64 *Q, *R;         // t: exclusive / inclusive prefix of btot inside each block's range
     s64 *Tseg;          // k: total of each block's range
     u64 *rlist;         // resolve list: minibucket << 32 | vertex
@@ -163,7 +164,6 @@ struct JetDev {
     s64 *evict_start;   // k + 1
     s64 *counter;       // strong-rebalance overflow position
     const s32 *D;       // k x k distances: mapping objective J (ProMap); null: edge cut
-    u32 count_bytes;    // hub kernels add their degree statistics (the byte accounting of the profiler)
 };
 
 __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64) d.vw[u] : 1; }
@@ -268,7 +268,12 @@ __device__ __forceinline__ GroupConn group_conn(char *smem, u32 k, u32 gidx) {
 // D / (stride * ADJ_UNROLL) rounds of dependent memory round trips instead of
 // D / stride (a hub's workgroup walks thousands of entries per thread group).
 constexpr u32 ADJ_UNROLL = 4;
+// unroll of a lane group of G lanes: narrow groups walk short adjacencies,
+// where the unrolled form only adds instructions and registers
+template<int G>
+constexpr u32 group_unroll() { return G >= 32 ? ADJ_UNROLL : 1; }
 
+template<u32 ADJ_UNROLL>
 __device__ __forceinline__ void conn_accumulate(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, u32 G) {
     const u64 b0 = d.xadj[u], e = d.xadj[u + 1];
     for (u64 j0 = b0 + lane; j0 < e; j0 += (u64) G * ADJ_UNROLL) {
@@ -648,7 +653,7 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
                     nev++;
                     degs += degree(d, u);
                 }
-                conn_accumulate(d, gc, u, lane, G);
+                conn_accumulate<group_unroll<G>()>(d, gc, u, lane, G);
             }
             wave_sync();
             const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, nullptr, active)
@@ -764,7 +769,7 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     for (u32 i = threadIdx.x; i < d.k; i += JB) gc.cnt[i] = 0;
     if (threadIdx.x == 0) *gc.tcount = 0;
     __syncthreads();
-    conn_accumulate(d, gc, u, threadIdx.x, JB);
+    conn_accumulate<ADJ_UNROLL>(d, gc, u, threadIdx.x, JB);
     __syncthreads();
     const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, ctl->ratio, red, red2, true)
                                   : reb_eval<JB>(d, strong, mbw, rs, gc, u, threadIdx.x, red, red2, true);
@@ -780,6 +785,7 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
 // ---------------------------------------------------------------- post: afterburner / classify
 // afterburner (jet:269-328): selected candidates are locked, stamped and
 // appended to the move list
+template<u32 ADJ_UNROLL>
 __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 lane, u32 stride) {
     const s64 ug = d.gain1[u];
     const u32 o = d.work[u], nn = d.dest[u];
@@ -829,7 +835,7 @@ __device__ __forceinline__ void afterburner_list(const JetDev &d, u32 stamp, con
         const bool active = i < len;
         const u32 u = active ? region[i] : 0;
         const bool hub = active && degree(d, u) > d.hub_deg;
-        s64 ch = (active && !hub) ? afterburner_change(d, u, lane, G) : 0;
+        s64 ch = (active && !hub) ? afterburner_change<group_unroll<G>()>(d, u, lane, G) : 0;
         ch = group_sum<G>(ch, (s64 *) nullptr);
         bool sel = false;
         if (active && !hub && lane == 0) {
@@ -1018,10 +1024,10 @@ __device__ void afterburner_flat(const JetDev &d, u32 stamp, const u32 *region, 
     }
 }
 
+template<int G, bool FLAT>
 __global__ __launch_bounds__(JB) void post_kernel(JetDev d) {
     __shared__ u32 s_cnt;
     __shared__ u64 s_red[JB / WAVE];
-    __shared__ FlatLds L;
     JetCtl *ctl = d.ctl;
     const u32 len = d.cand_n[blockIdx.x];  // issued with the control block: one round trip
     if (!live(ctl)) return;
@@ -1031,7 +1037,12 @@ __global__ __launch_bounds__(JB) void post_kernel(JetDev d) {
     __syncthreads();
     if (mode == 0) {
         u64 gh = 0, degs = 0;
-        afterburner_flat(d, stamp, d.cand + (u64) blockIdx.x * d.cap, len, mregion, &s_cnt, L, degs, gh);
+        if constexpr (FLAT) {
+            __shared__ FlatLds L;
+            afterburner_flat(d, stamp, d.cand + (u64) blockIdx.x * d.cap, len, mregion, &s_cnt, L, degs, gh);
+        } else {
+            afterburner_list<G>(d, stamp, d.cand + (u64) blockIdx.x * d.cap, len, mregion, &s_cnt, degs, gh);
+        }
         degs = block_total(degs, s_red);
         gh = block_total(gh, s_red);
         if (threadIdx.x == 0) {
@@ -1057,7 +1068,7 @@ __global__ __launch_bounds__(JB) void hub_post_kernel(JetDev d, const u32 *hubs)
     const u32 u = hubs[blockIdx.x];
     const s64 ug = d.gain1[u];
     if (ug == GAIN_MIN) return;  // not a candidate
-    s64 ch = afterburner_change(d, u, threadIdx.x, JB);
+    s64 ch = afterburner_change<ADJ_UNROLL>(d, u, threadIdx.x, JB);
     ch = group_sum<JB>(ch, red);
     bool sel = false;
     if (threadIdx.x == 0) {
@@ -1339,6 +1350,7 @@ __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
 // neighbours' old block comes from oldp, so concurrent writes to work[] are safe.
 // the change of the objective over edges [b, e) of moved vertex u (lanes
 // `lane`, `lane + stride`, ...), invalidating the caches of the neighbours
+template<u32 ADJ_UNROLL>
 __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u64 e, u32 lane, u32 stride,
                                                 u32 stamp) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
@@ -1386,8 +1398,9 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
     return delta;
 }
 
+template<u32 U>
 __device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 stride, u32 stamp) {
-    return move_delta_range(d, u, d.xadj[u], d.xadj[u + 1], lane, stride, stamp);
+    return move_delta_range<U>(d, u, d.xadj[u], d.xadj[u + 1], lane, stride, stamp);
 }
 
 // commits a move; the block weight changes go to `bw` (the global array, or a
@@ -1610,7 +1623,7 @@ __global__ __launch_bounds__(JB) void apply_hub_kernel(JetDev d, const u32 *hubs
     if (d.mark[u] != stamp) return;
     const u64 b = (u64) d.xadj[u] + (u64) ci * HUB_CHUNK, end = d.xadj[u + 1];
     const u64 e = b + HUB_CHUNK < end ? b + HUB_CHUNK : end;
-    s64 acc = move_delta_range(d, u, b, e, threadIdx.x, JB, stamp);
+    s64 acc = move_delta_range<ADJ_UNROLL>(d, u, b, e, threadIdx.x, JB, stamp);
     acc = block_sum<JB>(acc, red);
     if (threadIdx.x == 0) {
         if (acc) atomicAdd(&ctl->st[ST_DELTA], (unsigned long long) acc);
@@ -1641,13 +1654,13 @@ __global__ void hub_chunk_fill_kernel(const u32 *cnt, const u32 *off, u32 n_hubs
 // extra list
 template<int G>
 __device__ __forceinline__ void apply_one(const JetDev &d, u32 u, bool active, u32 lane, u32 stamp, bool do_hash,
-                                          s64 &acc, u64 &degs, u64 &mh) {
+                                          s64 &acc, u64 &degs, u64 &mh, s64 *bw) {
     const bool hub = active && degree(d, u) > d.hub_deg;
     if (active && !hub) {
-        acc += move_delta(d, u, lane, G, stamp);
+        acc += move_delta<group_unroll<G>()>(d, u, lane, G, stamp);
         if (lane == 0) {
             degs += degree(d, u);
-            move_commit(d, u);
+            move_commit(d, u, bw);
             if (do_hash) mh += mix64d(((u64) u << 32) | d.dest[u]);
         }
     }
@@ -1656,6 +1669,47 @@ __device__ __forceinline__ void apply_one(const JetDev &d, u32 u, bool active, u
 // apply (perform_moves, jet:753-924 / promap:1104-1216): this workgroup's
 // move region, then its contiguous share of the extra list, edge-balanced
 constexpr u32 BW_LDS = 1024;  // blocks whose weight changes an apply workgroup sums in LDS
+
+// lane-group form (levels without a skewed degree distribution): G lanes per
+// move, no staging or prefix
+template<int G>
+__global__ __launch_bounds__(JB) void apply_groups_kernel(JetDev d) {
+    __shared__ u64 s_red[JB / WAVE];
+    JetCtl *ctl = d.ctl;
+    const u32 groups = blockDim.x / G;
+    const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
+    const u32 *region = d.moves + (u64) blockIdx.x * d.cap_mv;
+    // issued with the control block (one round trip): the region length, the
+    // group's first entry (the region holds at least `groups` entries) and the
+    // extra list's length
+    const u32 len = d.moves_n[blockIdx.x];
+    const u32 first = region[gi];
+    const u32 lx = (u32) ctl->st[ST_MOVES];
+    if (!live(ctl)) return;
+    const u32 stamp = ctl->stamp;
+    const bool do_hash = d.trace != nullptr;
+    s64 *bwt = d.bw;  // few moves per workgroup here: direct updates
+    s64 acc = 0;
+    u64 mh = 0, degs = 0;
+    for (u32 i0 = 0; i0 < len; i0 += groups) {
+        const u32 i = i0 + gi;
+        apply_one<G>(d, i < len ? (i0 == 0 ? first : region[i]) : 0u, i < len, lane, stamp, do_hash, acc, degs, mh,
+                     bwt);
+    }
+    for (u64 i0 = (u64) blockIdx.x * groups; i0 < lx; i0 += (u64) gridDim.x * groups) {
+        const u64 i = i0 + gi;
+        apply_one<G>(d, i < lx ? d.moves_x[i] : 0u, i < lx, lane, stamp, do_hash, acc, degs, mh, bwt);
+    }
+    const u64 a = block_total((u64) acc, s_red);
+    degs = block_total(degs, s_red);
+    mh = block_total(mh, s_red);
+    if (threadIdx.x == 0) {
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_DELTA] = a;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_MOVE_DEG] = degs;
+        d.pstat[(u64) blockIdx.x * PT_COUNT + PT_MHASH] = mh;
+    }
+}
+
 __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
     __shared__ u64 s_red[JB / WAVE];
     __shared__ FlatLds L;
@@ -1763,13 +1817,8 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
 // partials (one barrier), then control_step
 __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
     constexpr u32 NV = PT_COUNT + 3;  // partials, two list lengths, max block weight
-    constexpr u32 CTL_WORDS = sizeof(JetCtl) / 8;
-    static_assert(sizeof(JetCtl) % 8 == 0 && CTL_WORDS <= JB, "control block layout");
     __shared__ u64 s_part[JB / WAVE][NV];
-    __shared__ u64 s_ctl[CTL_WORDS];
-    // the control block, the partials and the block weights are independent
-    // loads: issue them together (one memory round trip)
-    if (threadIdx.x < CTL_WORDS) s_ctl[threadIdx.x] = ((const u64 *) d.ctl)[threadIdx.x];
+    if (!live(d.ctl)) return;
     u64 t[NV];
 #pragma unroll
     for (u32 j = 0; j < NV; ++j) t[j] = 0;
@@ -1791,8 +1840,6 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const JetCtl &cur = *(const JetCtl *) s_ctl;
-        if (!cur.active || cur.stalled) return;
         s64 m = INT64_MIN;
 #pragma unroll
         for (u32 j = 0; j < NV - 1; ++j) t[j] = 0;
@@ -1801,7 +1848,7 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
             for (u32 j = 0; j < NV - 1; ++j) t[j] += s_part[w][j];
             m = (s64) s_part[w][NV - 1] > m ? (s64) s_part[w][NV - 1] : m;
         }
-        control_step(d, m, t, cur);
+        control_step(d, m, t, *d.ctl);
     }
 }
 
@@ -1913,7 +1960,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
                         nev++;
                         degs += degree(d, u);
                     }
-                    conn_accumulate(d, gc, u, lane, G);
+                    conn_accumulate<group_unroll<G>()>(d, gc, u, lane, G);
                 }
                 wave_sync();
                 const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, nullptr, active)
@@ -2024,7 +2071,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
         u64 mdeg = 0, mh = 0;
         for (u32 i0 = 0; i0 < nm; i0 += BM / G) {
             const u32 i = i0 + gi;
-            apply_one<G>(d, i < nm ? d.moves_x[i] : 0u, i < nm, lane, stamp, d.trace != nullptr, acc, mdeg, mh);
+            apply_one<G>(d, i < nm ? d.moves_x[i] : 0u, i < nm, lane, stamp, d.trace != nullptr, acc, mdeg, mh, d.bw);
         }
         const u64 a = block_total((u64) acc, s_red);
         mdeg = block_total(mdeg, s_red);
@@ -2080,6 +2127,9 @@ __global__ void ctl_init_kernel(JetDev d, RatioTable rt, s64 cut, s64 maxw) {
     ctl->iters = 0;
     ctl->n_trace = 0;
     ctl->stamp = 1;
+    double *ratios = (double *) d.ratios;
+#pragma unroll
+    for (u32 i = 0; i < MAX_RATIOS; ++i) ratios[i] = rt.r[i];
 }
 
 // ---------------------------------------------------------------- global resolve path
@@ -2197,6 +2247,7 @@ struct JetHost {
     size_t sweep_smem = 0, hub_smem = 0;
     s64 *H = nullptr, *roff = nullptr;
     u32 *over_flag = nullptr, *olist = nullptr, *olen = nullptr;
+    bool flat = false;  // edge-balanced afterburner and apply (skewed degree distribution)
     u32 n_chunks = 0;  // hub chunks of the apply step
     u32 *chunk_hub = nullptr, *chunk_idx = nullptr;
 
@@ -2211,8 +2262,18 @@ struct JetHost {
         if (n_chunks)
             launch(c, "jet_apply", 0.0, dim3(n_chunks), dim3(JB), 0, apply_hub_kernel, d, (const u32 *) meta.hubs,
                    (const u32 *) chunk_hub, (const u32 *) chunk_idx);
-        launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_kernel, d);
+        if (flat) launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_kernel, d);
+        else HEIPA_DISPATCH_G(G, launch_apply);
         launch(c, "jet_control", 0.0, dim3(1), dim3(JB), 0, control_kernel, d);
+    }
+
+    template<int GG>
+    void launch_post() {
+        launch(c, "jet_post", 0.0, dim3(d.nreg), dim3(JB), 0, post_kernel<GG, false>, d);
+    }
+    template<int GG>
+    void launch_apply() {
+        launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_groups_kernel<GG>, d);
     }
 
     template<int GG>
@@ -2231,7 +2292,8 @@ struct JetHost {
             launch(c, "jet_post", 0.0, dim3(meta.n_hubs), dim3(JB), 0, hub_post_kernel, d, (const u32 *) meta.hubs);
         }
         launch(c, "jet_buckets", 0.0, dim3(d.k), dim3(JB), 0, bucket_kernel, d);
-        launch(c, "jet_post", 0.0, dim3(d.nreg), dim3(JB), 0, post_kernel, d);
+        if (flat) launch(c, "jet_post", 0.0, dim3(d.nreg), dim3(JB), 0, post_kernel<4, true>, d);
+        else HEIPA_DISPATCH_G(G, launch_post);
         if (big_resolve)
             launch(c, "jet_resolve", 0.0, dim3(1), dim3(RB), 0, resolve_kernel<RESOLVE_CAP>, d);
         else
@@ -2340,6 +2402,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     char *ctl_mem = bs.get<char>(snap_bytes);
     d.ctl = (JetCtl *) ctl_mem;
     d.trace = tracing ? (heipa_jet_trace *) (ctl_mem + ctl_bytes) : nullptr;
+    d.ratios = bs.get<double>(MAX_RATIOS);
     RatioTable rt{};
     {
         const double ur[MAX_RATIOS] = {0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45,
@@ -2352,7 +2415,6 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
             rt.n = 1;
         }
         d.n_ratios = rt.n;
-        for (u32 i = 0; i < MAX_RATIOS; ++i) d.ratios[i] = rt.r[i];
     }
     h.big_resolve = c.max_lds >= 160 * 1024;
     // single-workgroup form for small levels without hubs
@@ -2383,6 +2445,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     // k-counter array per group, so wide k shrinks the groups per workgroup and,
     // if one wave's groups do not fit, widens the groups
     h.G = meta.G;
+    h.flat = meta.max_deg > 8u * meta.G;  // skewed: a lane group per vertex would wait on the heaviest ones
     {
         const size_t cgb = conn_group_bytes(k);
         require(cgb <= SWEEP_LDS, "k too large for the LDS block-connectivity counters");
