@@ -117,19 +117,29 @@ enum Stat : u32 {
 };
 
 // The state of jet_refine's loop (jet:960-1030), in device memory.
+// The head of the control block: every launch reads it at its start, in
+// one load issued before the live check (loads behind the check's early
+// return would be a second dependent memory round trip).
+struct CtlHead {
+    double ratio;  // ratio of the current ratio index
+    u32 active;    // 0 once the loop has ended
+    u32 stalled;   // the resolve list did not fit: the host runs the global path
+    u32 mode;
+    u32 save;      // the partition after the last iteration is the best so far
+    u32 stamp;
+    u32 zero_lp;   // mapping objective: LP iterations in a row without moves
+};
 struct JetCtl {
+    double ratio;
+    u32 active, stalled, mode, save, stamp, zero_lp;
     unsigned long long st[ST_COUNT];
     s64 cur_cut, cur_maxw, best_cut, best_maxw;
-    double ratio;     // ratio of the current ratio index
     double bytes[5];  // algorithmic bytes of the sweep, buckets, post, resolve and apply launches
-    u32 ri, it, bal, mode;
-    u32 active;       // 0 once the loop has ended
-    u32 stalled;      // the resolve list did not fit: the host runs the global path
-    u32 save;         // the partition after the last iteration is the best so far
+    u32 ri, it, bal;
     u32 iters;        // iterations that changed the partition
-    u32 n_trace, stamp;
-    u32 zero_lp;      // mapping objective: LP iterations in a row without moves
+    u32 n_trace;
 };
+static_assert(sizeof(CtlHead) == 32 && offsetof(JetCtl, st) == sizeof(CtlHead), "control block head layout");
 
 struct JetDev {
     u32 n, k;
@@ -178,6 +188,12 @@ __device__ __forceinline__ void btot_add(const JetDev &d, u32 g, s64 w) {
 __device__ __forceinline__ s64 ewt(const JetDev &d, u32 j) { return d.ew ? (s64) d.ew[j] : 1; }
 __device__ __forceinline__ u32 degree(const JetDev &d, u32 u) { return d.xadj[u + 1] - d.xadj[u]; }
 __device__ __forceinline__ bool live(const JetCtl *ctl) { return ctl->active && !ctl->stalled; }
+__device__ __forceinline__ CtlHead ctl_head(const JetCtl *ctl) {
+    CtlHead h;
+    __builtin_memcpy(&h, ctl, sizeof(h));
+    return h;
+}
+__device__ __forceinline__ bool live(const CtlHead &h) { return h.active && !h.stalled; }
 
 // eviction threshold of rebalance_strong (jet:363) / rebalance_weak (jet:600-604)
 __device__ __forceinline__ s64 max_block_weight(const JetDev &d, bool strong) {
@@ -586,12 +602,12 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
     extern __shared__ __align__(16) char smem[];
     __shared__ u32 s_cnt, s_nlist;
     __shared__ u64 s_red[JB / WAVE];
-    JetCtl *ctl = d.ctl;
-    if (!live(ctl)) return;
+    const CtlHead hd = ctl_head(d.ctl);
+    if (!live(hd)) return;
     u32 *region = d.cand + (u64) blockIdx.x * d.cap;
-    const u32 mode = ctl->mode;
-    const bool save = ctl->save != 0;
-    const double ratio = ctl->ratio;
+    const u32 mode = hd.mode;
+    const bool save = hd.save != 0;
+    const double ratio = hd.ratio;
     const u32 groups = blockDim.x / G;
     const u32 lane = threadIdx.x % G, gi = threadIdx.x / G;
     const size_t cgb = conn_group_bytes(d.k);
@@ -686,8 +702,8 @@ constexpr u32 BKT_UNROLL = 8;  // copies loaded per round (loads in flight per t
 __global__ __launch_bounds__(JB) void bucket_kernel(JetDev d) {
     __shared__ s64 red[JB / WAVE];
     __shared__ unsigned long long acc[BKT_CHUNK];
-    JetCtl *ctl = d.ctl;
-    if (!live(ctl) || ctl->mode == 0) return;
+    const CtlHead hd = ctl_head(d.ctl);
+    if (!live(hd) || hd.mode == 0) return;
     const u32 p = blockIdx.x;
     const u64 base = (u64) p * d.width;
     const u64 stride = (u64) d.t + 1;
@@ -751,8 +767,9 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     __shared__ Best red[JB / WAVE];
     __shared__ s64 red2[JB / WAVE];
     JetCtl *ctl = d.ctl;
-    if (!live(ctl)) return;
-    const u32 mode = ctl->mode;
+    const CtlHead hd = ctl_head(ctl);
+    if (!live(hd)) return;
+    const u32 mode = hd.mode;
     if ((mode == 0) != (phase == 1)) return;
     const u32 u = hubs[blockIdx.x];
     GroupConn gc = group_conn(smem, d.k, 0);
@@ -771,7 +788,7 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     __syncthreads();
     conn_accumulate<ADJ_UNROLL>(d, gc, u, threadIdx.x, JB);
     __syncthreads();
-    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, ctl->ratio, red, red2, true)
+    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, hd.ratio, red, red2, true)
                                   : reb_eval<JB>(d, strong, mbw, rs, gc, u, threadIdx.x, red, red2, true);
     append(listed, u, d.cand_x, &ctl->st[ST_CAND]);
     if (threadIdx.x == 0) {
@@ -1030,8 +1047,9 @@ __global__ __launch_bounds__(JB) void post_kernel(JetDev d) {
     __shared__ u64 s_red[JB / WAVE];
     JetCtl *ctl = d.ctl;
     const u32 len = d.cand_n[blockIdx.x];  // issued with the control block: one round trip
-    if (!live(ctl)) return;
-    const u32 mode = ctl->mode, stamp = ctl->stamp;
+    const CtlHead hd = ctl_head(ctl);
+    if (!live(hd)) return;
+    const u32 mode = hd.mode, stamp = hd.stamp;
     u32 *mregion = d.moves + (u64) blockIdx.x * d.cap_mv;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
@@ -1063,9 +1081,10 @@ __global__ __launch_bounds__(JB) void post_kernel(JetDev d) {
 __global__ __launch_bounds__(JB) void hub_post_kernel(JetDev d, const u32 *hubs) {
     __shared__ s64 red[JB / WAVE];
     JetCtl *ctl = d.ctl;
-    if (!live(ctl) || ctl->mode != 0) return;
-    const u32 stamp = ctl->stamp;
+    const CtlHead hd = ctl_head(ctl);
     const u32 u = hubs[blockIdx.x];
+    if (!live(hd) || hd.mode != 0) return;
+    const u32 stamp = hd.stamp;
     const s64 ug = d.gain1[u];
     if (ug == GAIN_MIN) return;  // not a candidate
     s64 ch = afterburner_change<ADJ_UNROLL>(d, u, threadIdx.x, JB);
@@ -1338,9 +1357,11 @@ template<u32 CAP>
 __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
     __shared__ __align__(16) char smem[resolve_lds_bytes(CAP)];
     JetCtl *ctl = d.ctl;
-    if (!live(ctl) || ctl->mode == 0) return;
+    const CtlHead hd = ctl_head(ctl);
+    const u32 L = (u32) ctl->st[ST_RLIST];  // the same cache line as the head
+    if (!live(hd) || hd.mode == 0) return;
     const ResolveLds lds = resolve_lds(smem, CAP);
-    const bool ok = resolve_body<CAP>(d, ctl->mode == 2, ctl->stamp, (u32) ctl->st[ST_RLIST], lds, nullptr);
+    const bool ok = resolve_body<CAP>(d, hd.mode == 2, hd.stamp, L, lds, nullptr);
     if (!ok && threadIdx.x == 0) ctl->stalled = 1;
 }
 
@@ -1617,8 +1638,9 @@ __global__ __launch_bounds__(JB) void apply_hub_kernel(JetDev d, const u32 *hubs
     __shared__ s64 red[JB / WAVE];
     JetCtl *ctl = d.ctl;
     const u32 h = chunk_hub[blockIdx.x], ci = chunk_idx[blockIdx.x];
-    if (!live(ctl)) return;
-    const u32 stamp = ctl->stamp;
+    const CtlHead hd = ctl_head(ctl);
+    if (!live(hd)) return;
+    const u32 stamp = hd.stamp;
     const u32 u = hubs[h];
     if (d.mark[u] != stamp) return;
     const u64 b = (u64) d.xadj[u] + (u64) ci * HUB_CHUNK, end = d.xadj[u + 1];
@@ -1685,8 +1707,9 @@ __global__ __launch_bounds__(JB) void apply_groups_kernel(JetDev d) {
     const u32 len = d.moves_n[blockIdx.x];
     const u32 first = region[gi];
     const u32 lx = (u32) ctl->st[ST_MOVES];
-    if (!live(ctl)) return;
-    const u32 stamp = ctl->stamp;
+    const CtlHead hd = ctl_head(ctl);
+    if (!live(hd)) return;
+    const u32 stamp = hd.stamp;
     const bool do_hash = d.trace != nullptr;
     s64 *bwt = d.bw;  // few moves per workgroup here: direct updates
     s64 acc = 0;
@@ -1719,8 +1742,9 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
     // issued with the control block (one round trip)
     const u32 len = d.moves_n[blockIdx.x];
     const u32 lx = (u32) ctl->st[ST_MOVES];
-    if (!live(ctl)) return;
-    const u32 stamp = ctl->stamp;
+    const CtlHead hd = ctl_head(ctl);
+    if (!live(hd)) return;
+    const u32 stamp = hd.stamp;
     const bool do_hash = d.trace != nullptr;
     const u32 xch = (lx + gridDim.x - 1) / gridDim.x;
     const u32 x0 = blockIdx.x * xch < lx ? blockIdx.x * xch : lx;
@@ -1814,11 +1838,16 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
 }
 
 // one workgroup: the largest block weight and the sums of the per-workgroup
-// partials (one barrier), then control_step
+// partials (one barrier), then control_step.  The control block is copied to
+// LDS with the first loads, so neither the live check nor control_step adds
+// a memory round trip.
 __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
     constexpr u32 NV = PT_COUNT + 3;  // partials, two list lengths, max block weight
+    constexpr u32 CW = (sizeof(JetCtl) + 7) / 8;
+    static_assert(CW <= JB && sizeof(JetCtl) % 8 == 0, "control block copy");
     __shared__ u64 s_part[JB / WAVE][NV];
-    if (!live(d.ctl)) return;
+    __shared__ u64 s_ctl[CW];
+    if (threadIdx.x < CW) s_ctl[threadIdx.x] = ((const u64 *) d.ctl)[threadIdx.x];
     u64 t[NV];
 #pragma unroll
     for (u32 j = 0; j < NV; ++j) t[j] = 0;
@@ -1839,7 +1868,8 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
         s_part[threadIdx.x / WAVE][NV - 1] = (u64) mx;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    const JetCtl &in = *(const JetCtl *) s_ctl;
+    if (threadIdx.x == 0 && in.active && !in.stalled) {
         s64 m = INT64_MIN;
 #pragma unroll
         for (u32 j = 0; j < NV - 1; ++j) t[j] = 0;
@@ -1848,7 +1878,7 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
             for (u32 j = 0; j < NV - 1; ++j) t[j] += s_part[w][j];
             m = (s64) s_part[w][NV - 1] > m ? (s64) s_part[w][NV - 1] : m;
         }
-        control_step(d, m, t, *d.ctl);
+        control_step(d, m, t, in);
     }
 }
 
@@ -1894,11 +1924,12 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
     for (u64 j = tid; j < (u64) d.nreg * PT_COUNT; j += BM) d.pstat[j] = 0;
     for (u32 iter = 0; iter < max_iters; ++iter) {
         if (tid == 0) {
-            s_live = live(ctl);
-            s_mode = ctl->mode;
-            s_stamp = ctl->stamp;
-            s_save = ctl->save;
-            s_ratio = ctl->ratio;
+            const CtlHead hd = ctl_head(ctl);
+            s_live = live(hd);
+            s_mode = hd.mode;
+            s_stamp = hd.stamp;
+            s_save = hd.save;
+            s_ratio = hd.ratio;
             s_ncand = s_nmoves = s_nrl = 0;
             for (u32 j = 0; j < PT_COUNT; ++j) s_acc[j] = 0;
         }
