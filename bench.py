@@ -119,6 +119,15 @@ def main():
                   f"GB/s={s['bytes'] / max(s['ms'], 1e-9) / 1e6:9.1f}", file=sys.stderr)
     accounted = {n: s for n, s in prof_all.items() if s["bytes"] > 0}
     dom = max(accounted, key=lambda n: accounted[n]["ms"]) if accounted else None
+    # every named launch group of the profiled step (events around every launch,
+    # so short launches read slower than in the timed steps), largest first
+    per_kernel = {}
+    for name, s in sorted(prof_all.items(), key=lambda kv: -kv[1]["ms"])[:10]:
+        gbs = s["bytes"] / max(s["ms"], 1e-9) / 1e6 if s["bytes"] > 0 else None
+        per_kernel[name] = {"launches": s["launches"], "ms": round(s["ms"], 3),
+                            "avg_us": round(s["ms"] * 1000.0 / max(s["launches"], 1), 3),
+                            "alg_GBps": None if gbs is None else round(gbs, 1),
+                            "frac": None if gbs is None else round(gbs / 8000.0, 5)}
 
     # timed region: K steps; HIP events bracket about one launch in 8 of the
     # dominant kernel, chosen by a hash of the launch count (its bytes are counted
@@ -244,6 +253,7 @@ def main():
                        "refinement": round(stats.ms_refinement, 2)}},
         "quality": quality,
         "roofline": roof,
+        "per_kernel": per_kernel,
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
     }
