@@ -240,6 +240,24 @@ __device__ __forceinline__ void append_local(bool pred, u32 val, u32 *region, u3
     if (pred) region[base + (u32) __popcll(bal & lanemask_lt())] = val;
 }
 
+// append_local of a vertex with two more words at the same index of two
+// parallel arrays
+__device__ __forceinline__ void append_local3(bool pred, u32 val, u32 a, u32 b, u32 *region, u32 *ra, u32 *rb,
+                                              u32 *s_cnt) {
+    const u64 bal = __ballot(pred);
+    if (!bal) return;
+    const u32 first = (u32) __builtin_ctzll(bal);
+    u32 base = 0;
+    if (lane_id() == first) base = atomicAdd(s_cnt, (u32) __popcll(bal));
+    base = __shfl(base, (int) first, WAVE);
+    if (pred) {
+        const u32 i = base + (u32) __popcll(bal & lanemask_lt());
+        region[i] = val;
+        ra[i] = a;
+        rb[i] = b;
+    }
+}
+
 // sum over a workgroup of any multiple-of-64 size; the result is valid in
 // thread 0; `lds` holds blockDim.x / 64 entries
 __device__ __forceinline__ u64 block_total(u64 v, u64 *lds) {
@@ -290,8 +308,8 @@ template<int G>
 constexpr u32 group_unroll() { return G >= 32 ? ADJ_UNROLL : 1; }
 
 template<u32 ADJ_UNROLL>
-__device__ __forceinline__ void conn_accumulate(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, u32 G) {
-    const u64 b0 = d.xadj[u], e = d.xadj[u + 1];
+__device__ __forceinline__ void conn_accumulate_range(const JetDev &d, const GroupConn &gc, u64 b0, u64 e, u32 lane,
+                                                      u32 G) {
     for (u64 j0 = b0 + lane; j0 < e; j0 += (u64) G * ADJ_UNROLL) {
         u32 v[ADJ_UNROLL], bl[ADJ_UNROLL];
         s64 w[ADJ_UNROLL];
@@ -311,6 +329,11 @@ __device__ __forceinline__ void conn_accumulate(const JetDev &d, const GroupConn
             }
         }
     }
+}
+
+template<u32 ADJ_UNROLL>
+__device__ __forceinline__ void conn_accumulate(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, u32 G) {
+    conn_accumulate_range<ADJ_UNROLL>(d, gc, d.xadj[u], d.xadj[u + 1], lane, G);
 }
 
 __device__ __forceinline__ void conn_reset(const GroupConn &gc, u32 T, u32 lane, u32 G) {
@@ -376,10 +399,9 @@ __device__ __forceinline__ T group_sum(T x, T *red) {
 // best_block (:200-236) + filter (:247-258) of one rebuilt vertex; lane 0 of
 // the group writes the results; returns whether u is a candidate
 template<int G>
-__device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u32 u, u32 lane, double ratio,
-                                        Best *red, s64 *red2, bool active) {
+__device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u32 u, u32 own_b, bool locked, u32 lane,
+                                        double ratio, Best *red, s64 *red2, bool active) {
     const u32 T = active ? *gc.tcount : 0;
-    const u32 own_b = active ? d.work[u] : 0;
     Best best{0, NO_MOVE};
     if (d.D) {
         // ProMap_jet_lp best_block (promap_jet_label_propagation.h:230-258):
@@ -434,7 +456,7 @@ __device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u3
         d.gcache[u] = gain;
         d.dcache[u] = best_id;
         d.dest[u] = best_id;
-        if (best_id != NO_MOVE && d.lock[u] == 0) {
+        if (best_id != NO_MOVE && !locked) {
             d.gain1[u] = gain;
             cand = true;
         } else {
@@ -494,9 +516,9 @@ __device__ __forceinline__ bool reb_eligible(const JetDev &d, bool strong, const
 // u is a candidate
 template<int G>
 __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, const RebShared *rs,
-                                         const GroupConn &gc, u32 u, u32 lane, Best *red, s64 *red2, bool active) {
+                                         const GroupConn &gc, u32 u, u32 own_b, u32 lane, Best *red, s64 *red2,
+                                         bool active) {
     const u32 T = active ? *gc.tcount : 0;
-    const u32 own_b = active ? d.work[u] : 0;
     bool listed = false;
     if (!strong) {
         Best best{0, NO_MOVE};
@@ -612,8 +634,11 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
     const u32 lane = threadIdx.x % G, gi = threadIdx.x / G;
     const size_t cgb = conn_group_bytes(d.k);
     GroupConn gc = group_conn(smem, d.k, gi);
+    // listed vertices to rebuild, each with its adjacency start and a word
+    // own block << 16 | locked << 15 | degree (degree <= hub_deg <= 2048, k < 4096)
     u32 *list = (u32 *) (smem + cgb * groups);
-    RebShared *rs = (RebShared *) (smem + cgb * groups + 4ull * SWEEP_PIECE);
+    u32 *lbeg = list + SWEEP_PIECE, *linfo = lbeg + SWEEP_PIECE;
+    RebShared *rs = (RebShared *) (smem + cgb * groups + 12ull * SWEEP_PIECE);
     for (u32 i = lane; i < d.k; i += G) gc.cnt[i] = 0;
     if (lane == 0) *gc.tcount = 0;
     if (threadIdx.x == 0) s_cnt = 0;
@@ -632,13 +657,18 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
             const u64 vv = v0 + threadIdx.x;
             const u32 v = (u32) vv;
             bool heavy = false, cand = false;
+            u32 hb = 0, hinfo = 0;
             if (vv < p1) {
                 if (save) d.part[v] = d.work[v];
                 if (mode == 0) {
                     const u32 dc = d.dcache[v];
                     const u8 lk = d.lock[v];  // loaded with the cache entry: one round trip
                     if (dc == NULL_PART) {
-                        heavy = degree(d, v) <= d.hub_deg;
+                        // offsets and own block in one round trip, kept for phase B
+                        const u32 xb = d.xadj[v], xe = d.xadj[v + 1], own = d.work[v];
+                        heavy = xe - xb <= d.hub_deg;
+                        hb = xb;
+                        hinfo = own << 16 | (lk ? 1u << 15 : 0u) | (xe - xb);
                     } else {
                         d.dest[v] = dc;
                         if (dc != NO_MOVE && lk == 0) {
@@ -649,14 +679,22 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
                             d.lock[v] = 0;
                         }
                     }
-                } else if (const u32 wv = d.work[v]; degree(d, v) <= d.hub_deg && reb_eligible(d, strong, rs, v, wv)) {
-                    const s64 ic = d.icache[v];
-                    if (ic >= 0) cand = reb_interior(d, strong, rs, v, ic);
-                    else heavy = true;
+                } else {
+                    const u32 wv = d.work[v], xb = d.xadj[v], xe = d.xadj[v + 1];
+                    if (xe - xb <= d.hub_deg && reb_eligible(d, strong, rs, v, wv)) {
+                        const s64 ic = d.icache[v];
+                        if (ic >= 0) {
+                            cand = reb_interior(d, strong, rs, v, ic);
+                        } else {
+                            heavy = true;
+                            hb = xb;
+                            hinfo = wv << 16 | (xe - xb);
+                        }
+                    }
                 }
             }
             append_local(cand, v, region, &s_cnt);
-            append_local(heavy, v, list, &s_nlist);
+            append_local3(heavy, v, hb, hinfo, list, lbeg, linfo, &s_nlist);
         }
         __syncthreads();
         const u32 nl = s_nlist;
@@ -664,16 +702,19 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
             const u32 idx = r0 + gi;
             const bool active = idx < nl;
             const u32 u = active ? list[idx] : 0;
+            const u32 ub = active ? lbeg[idx] : 0, info = active ? linfo[idx] : 0;
+            const u32 own = info >> 16, deg = info & 0x7FFFu;
             if (active) {
                 if (lane == 0) {
                     nev++;
-                    degs += degree(d, u);
+                    degs += deg;
                 }
-                conn_accumulate<group_unroll<G>()>(d, gc, u, lane, G);
+                conn_accumulate_range<group_unroll<G>()>(d, gc, ub, (u64) ub + deg, lane, G);
             }
             wave_sync();
-            const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, nullptr, active)
-                                          : reb_eval<G>(d, strong, mbw, rs, gc, u, lane, nullptr, nullptr, active);
+            const bool listed = mode == 0
+                                    ? lp_eval<G>(d, gc, u, own, (info >> 15) & 1u, lane, ratio, nullptr, nullptr, active)
+                                    : reb_eval<G>(d, strong, mbw, rs, gc, u, own, lane, nullptr, nullptr, active);
             wave_sync();
             if (active) conn_reset(gc, *gc.tcount, lane, G);
             wave_sync();
@@ -786,10 +827,12 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     for (u32 i = threadIdx.x; i < d.k; i += JB) gc.cnt[i] = 0;
     if (threadIdx.x == 0) *gc.tcount = 0;
     __syncthreads();
+    const u32 own = d.work[u];
+    const bool locked = d.lock[u] != 0;
     conn_accumulate<ADJ_UNROLL>(d, gc, u, threadIdx.x, JB);
     __syncthreads();
-    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, threadIdx.x, hd.ratio, red, red2, true)
-                                  : reb_eval<JB>(d, strong, mbw, rs, gc, u, threadIdx.x, red, red2, true);
+    const bool listed = mode == 0 ? lp_eval<JB>(d, gc, u, own, locked, threadIdx.x, hd.ratio, red, red2, true)
+                                  : reb_eval<JB>(d, strong, mbw, rs, gc, u, own, threadIdx.x, red, red2, true);
     append(listed, u, d.cand_x, &ctl->st[ST_CAND]);
     if (threadIdx.x == 0) {
         if (d.count_bytes) {  // one update per hub workgroup of two counters: only when profiling
@@ -1994,8 +2037,11 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
                     conn_accumulate<group_unroll<G>()>(d, gc, u, lane, G);
                 }
                 wave_sync();
-                const bool listed = mode == 0 ? lp_eval<G>(d, gc, u, lane, ratio, nullptr, nullptr, active)
-                                              : reb_eval<G>(d, strong, mbw, rs, gc, u, lane, nullptr, nullptr, active);
+                const u32 own = active ? d.work[u] : 0;
+                const bool locked = active && d.lock[u] != 0;
+                const bool listed = mode == 0
+                                        ? lp_eval<G>(d, gc, u, own, locked, lane, ratio, nullptr, nullptr, active)
+                                        : reb_eval<G>(d, strong, mbw, rs, gc, u, own, lane, nullptr, nullptr, active);
                 wave_sync();
                 if (active) conn_reset(gc, *gc.tcount, lane, G);
                 wave_sync();
@@ -2486,9 +2532,10 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         while (gpb > WAVE / Gs && gpb * cgb > SWEEP_LDS) gpb >>= 1;
         h.Gs = Gs;
         h.sweep_threads = gpb * Gs;
-        h.sweep_smem = cgb * gpb + 4ull * SWEEP_PIECE + 16 + 4ull * k;
+        h.sweep_smem = cgb * gpb + 12ull * SWEEP_PIECE + 16 + 4ull * k;
         h.hub_smem = cgb + 16 + 4ull * k;
-        require(h.sweep_smem <= 65536 && h.hub_smem <= 65536, "k too large for the LDS block-connectivity counters");
+        require(h.sweep_smem <= c.max_lds && h.hub_smem <= 65536 && k < 4096,
+                "k too large for the LDS block-connectivity counters");
     }
     // list regions: sweep workgroup r owns vertices [r * cap, (r + 1) * cap),
     // cap a multiple of the workgroup size; its post workgroup may add its
