@@ -1247,6 +1247,35 @@ __device__ inline ResolveLds resolve_lds(char *base, u32 cap) {
     return r;
 }
 
+// Bitonic sort of up to RB keys ascending into skey[0, P2) (P2 a power of
+// two, entries from L on padded with ~0), key i in thread i's register:
+// stages whose partner lies in the same wave exchange by lane shuffles
+// without a barrier, only stages across waves go through LDS.  For longer
+// lists the LDS form is faster: a shuffle costs about as much as an LDS
+// access, and the LDS form skips the idle threads' work.
+__device__ __forceinline__ void sort_keys_one_per_thread(const u64 *src, u32 L, u32 P2, u64 *skey) {
+    const u32 i = threadIdx.x;
+    u64 r = i < L ? src[i] : ~0ull;
+    for (u32 kk = 2; kk <= P2; kk <<= 1) {
+        for (u32 j = kk >> 1; j > 0; j >>= 1) {
+            u64 b;
+            if (j >= WAVE) {
+                __syncthreads();  // the previous LDS stage has read its partners
+                if (i < P2) skey[i] = r;
+                __syncthreads();
+                b = i < P2 ? skey[i ^ j] : r;
+            } else {
+                b = __shfl_xor(r, (int) j);
+            }
+            const bool take_min = ((i & j) == 0) == ((i & kk) == 0);
+            r = take_min ? (b < r ? b : r) : (b > r ? b : r);
+        }
+    }
+    __syncthreads();
+    if (i < P2) skey[i] = r;
+    __syncthreads();
+}
+
 // Resolves a list of L entries with RB threads; false: the list (or the
 // strong overflow) does not fit and nothing was appended.  Moves go to the
 // extra move list, counted in LDS (`local_cnt`) or in st[ST_MOVES].
@@ -1266,6 +1295,9 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
     if (threadIdx.x == 0) ocount = 0;
     if (strong)
         for (u32 b = threadIdx.x; b < d.k; b += RB) d.evict_adjust[b] = 0;
+    if (P2 <= RB) {
+        sort_keys_one_per_thread(d.rlist, L, P2, skey);
+    } else {
     for (u32 i = threadIdx.x; i < P2; i += RB) skey[i] = i < L ? d.rlist[i] : ~0ull;
     __syncthreads();
     // bitonic sort, ascending
@@ -1283,6 +1315,7 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
             }
             __syncthreads();
         }
+    }
     }
     // exclusive prefix of the weights in sorted order
     s64 carry = 0;
