@@ -437,6 +437,7 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         else if (option == HEIPA_OPT_GRAPHS) c.use_graphs = value != 0;
         else if (option == HEIPA_OPT_PROF_SAMPLE) c.prof_sample = value < 1 ? 1u : (u32) std::min<uint64_t>(value, 1u << 30);
         else if (option == HEIPA_OPT_BUCKET_COPIES) c.bucket_copies = (u32) std::min<uint64_t>(value, 32);
+        else if (option == HEIPA_OPT_BLOCK_LDS) c.block_lds = value != 0;
         else require(false, "unknown option");
     });
 }

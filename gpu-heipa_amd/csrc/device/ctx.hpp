@@ -153,9 +153,10 @@ struct Ctx {
     int num_cus = 256;
     size_t max_lds = 65536;  // LDS bytes one workgroup may allocate
     u32 resolve_cap = ~0u;   // HEIPA_OPT_RESOLVE_CAP
-    u32 block_jet_max_n = 1024;  // HEIPA_OPT_BLOCK_JET
+    u32 block_jet_max_n = 2048;  // HEIPA_OPT_BLOCK_JET
     bool use_graphs = true;  // HEIPA_OPT_GRAPHS
     u32 bucket_copies = 0;   // HEIPA_OPT_BUCKET_COPIES (0: automatic)
+    bool block_lds = true;   // HEIPA_OPT_BLOCK_LDS
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;

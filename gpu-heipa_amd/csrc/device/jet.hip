@@ -1977,9 +1977,50 @@ __host__ __device__ inline size_t bm_sweep_lds_bytes(u32 k, u32 groups) {
     return conn_group_bytes(k) * groups + 4ull * BM_PIECE + 16 + 4ull * k;
 }
 
-template<int G>
-__global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, u32 groups) {
+// LDS copy of the per-vertex state of the single-workgroup form: s64
+// gcache, gain1, icache (n each) and the k block weights, then u32 work,
+// dest, dcache, mark, oldp, gid (n each), then the u8 locks
+__host__ __device__ constexpr size_t bm_state_bytes(u32 n, u32 k) { return 49ull * n + 8ull * k; }
+
+// LS: the per-vertex state lives in LDS (from byte state_off of the dynamic
+// LDS) for the whole launch: copied in at the start and back to device
+// memory before the kernel returns (the host's global resolve path and the
+// next launch read it there).  The device functions reach it through the
+// same pointers in a local copy of the Jet state, as generic addresses.
+template<int G, bool LS>
+__global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters, u32 groups, u32 state_off) {
     extern __shared__ __align__(16) char smem[];
+    JetDev d = dg;
+    if constexpr (LS) {
+        const u32 n = dg.n, k = dg.k;
+        s64 *gc8 = (s64 *) (smem + state_off);
+        d.gcache = gc8;
+        d.gain1 = gc8 + n;
+        d.icache = gc8 + 2ull * n;
+        d.bw = gc8 + 3ull * n;
+        u32 *w4 = (u32 *) (gc8 + 3ull * n + k);
+        d.work = w4;
+        d.dest = w4 + n;
+        d.dcache = w4 + 2ull * n;
+        d.mark = w4 + 3ull * n;
+        d.oldp = w4 + 4ull * n;
+        d.gid = w4 + 5ull * n;
+        d.lock = (u8 *) (w4 + 6ull * n);
+        for (u32 v = threadIdx.x; v < n; v += BM) {
+            d.gcache[v] = dg.gcache[v];
+            d.gain1[v] = dg.gain1[v];
+            d.icache[v] = dg.icache[v];
+            d.work[v] = dg.work[v];
+            d.dest[v] = dg.dest[v];
+            d.dcache[v] = dg.dcache[v];
+            d.mark[v] = dg.mark[v];
+            d.oldp[v] = dg.oldp[v];
+            d.gid[v] = dg.gid[v];
+            d.lock[v] = dg.lock[v];
+        }
+        for (u32 b = threadIdx.x; b < k; b += BM) d.bw[b] = dg.bw[b];
+        __syncthreads();
+    }
     __shared__ u32 s_live, s_mode, s_stamp, s_save;
     __shared__ double s_ratio;
     __shared__ u32 s_ncand, s_nlist, s_nmoves, s_nrl;
@@ -2208,6 +2249,22 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev d, u32 max_iters, 
         }
         __syncthreads();
     }
+    if constexpr (LS) {
+        __syncthreads();
+        for (u32 v = threadIdx.x; v < dg.n; v += BM) {
+            dg.gcache[v] = d.gcache[v];
+            dg.gain1[v] = d.gain1[v];
+            dg.icache[v] = d.icache[v];
+            dg.work[v] = d.work[v];
+            dg.dest[v] = d.dest[v];
+            dg.dcache[v] = d.dcache[v];
+            dg.mark[v] = d.mark[v];
+            dg.oldp[v] = d.oldp[v];
+            dg.gid[v] = d.gid[v];
+            dg.lock[v] = d.lock[v];
+        }
+        for (u32 b = threadIdx.x; b < dg.k; b += BM) dg.bw[b] = d.bw[b];
+    }
 }
 
 struct RatioTable {
@@ -2354,6 +2411,8 @@ struct JetHost {
     bool block_mode = false;
     u32 bm_groups = 1;
     size_t bm_lds = 0;
+    bool bm_ls = false;  // single-workgroup form with the per-vertex state in LDS
+    u32 bm_state_off = 0;
     size_t sweep_smem = 0, hub_smem = 0;
     s64 *H = nullptr, *roff = nullptr;
     u32 *over_flag = nullptr, *olist = nullptr, *olen = nullptr;
@@ -2388,7 +2447,12 @@ struct JetHost {
 
     template<int GG>
     void launch_block(u32 max_iters) {
-        launch(c, "jet_block", 0.0, dim3(1), dim3(BM), bm_lds, jet_block_kernel<GG>, d, max_iters, bm_groups);
+        if (bm_ls)
+            launch(c, "jet_block", 0.0, dim3(1), dim3(BM), bm_state_off + bm_state_bytes(d.n, d.k),
+                   jet_block_kernel<GG, true>, d, max_iters, bm_groups, bm_state_off);
+        else
+            launch(c, "jet_block", 0.0, dim3(1), dim3(BM), bm_lds, jet_block_kernel<GG, false>, d, max_iters, bm_groups,
+                   0u);
     }
 
     void iteration() {
@@ -2536,6 +2600,10 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         // (coarse levels of skewed graphs) stay on the multi-launch form
         h.block_mode = n <= c.block_jet_max_n && (u64) g.m <= 32ull * c.block_jet_max_n && meta.n_hubs == 0 &&
                        h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
+        if (h.block_mode) {
+            h.bm_state_off = (u32) ((h.bm_lds + 15) & ~(size_t) 15);
+            h.bm_ls = c.block_lds && h.bm_state_off + bm_state_bytes(n, k) <= c.max_lds;
+        }
     }
     // copies of the minibucket totals: about one per 64 sweep workgroups, at
     // most 32 and 8 MiB; the single-workgroup form keeps one

@@ -218,7 +218,8 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
  * both paths. */
 #define HEIPA_OPT_RESOLVE_CAP 1
 /* HEIPA_OPT_BLOCK_JET: the largest level (vertices) whose Jet runs in one workgroup when it has no
- * hubs (default 1024; 0 forces the multi-launch form everywhere).  Results do not depend on it. */
+ * hubs and at most 32 edges per vertex on average (default 2048; 0 forces the multi-launch form
+ * everywhere).  Results do not depend on it. */
 #define HEIPA_OPT_BLOCK_JET 2
 /* HEIPA_OPT_GRAPHS: 1 (default) replays each level's batch of Jet iterations as a captured HIP graph;
  * 0 launches every kernel.  Results do not depend on it. */
@@ -231,6 +232,10 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
  * spreads its atomic additions over (0 = automatic, default: about one per 64 sweep workgroups, at
  * most 32; otherwise rounded down to a power of two, at most 32).  Results do not depend on it. */
 #define HEIPA_OPT_BUCKET_COPIES 5
+/* HEIPA_OPT_BLOCK_LDS: 1 (default) keeps the per-vertex Jet state of a single-workgroup level in LDS
+ * for the whole launch when it fits (copied in at the start, back at the end); 0 keeps it in device
+ * memory.  Results do not depend on it. */
+#define HEIPA_OPT_BLOCK_LDS 6
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */

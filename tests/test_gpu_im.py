@@ -68,8 +68,22 @@ def test_map_refine_multi_launch(ctx, graph, graphs, hier, skew):
     try:
         test_map_refine(ctx, graph, hier, skew, 1.03)
     finally:
-        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
         ctx.set_option(heipa.OPT_GRAPHS, 1)
+
+
+@pytest.mark.parametrize("lds", [1, 0])
+def test_map_refine_block_state(ctx, lds):
+    """The mapping objective in the single-workgroup form, per-vertex state in
+    LDS or in device memory."""
+    ctx.set_option(heipa.OPT_BLOCK_JET, 1 << 20)
+    ctx.set_option(heipa.OPT_BLOCK_LDS, lds)
+    try:
+        for hier, skew in (("4:8:6", 0.0), ("2:2:2", 0.9)):
+            test_map_refine(ctx, ("grid40", heipa.gen_grid(40, 40)), hier, skew, 1.03)
+    finally:
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
+        ctx.set_option(heipa.OPT_BLOCK_LDS, 1)
 
 
 @pytest.mark.parametrize("cap", [0, 16])

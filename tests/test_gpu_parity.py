@@ -133,7 +133,7 @@ def test_jet_multi_launch(ctx, graph, graphs, ultra, level, skew, k):
     try:
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
-        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
         ctx.set_option(heipa.OPT_GRAPHS, 1)
 
 
@@ -150,7 +150,32 @@ def test_jet_global_resolve(ctx, graph, cap, block, ultra, level, skew, k):
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
         ctx.set_option(heipa.OPT_RESOLVE_CAP, 1 << 32)
-        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
+
+
+SMALL = {"grid40": lambda: heipa.gen_grid(40, 40), "rgg2kw": lambda: heipa.gen_rgg(2000, 10.0, 4, 10),
+         "rmat11u": lambda: GRAPHS["rmat11u"], "rgg12vw": lambda: GRAPHS["rgg12vw"]}
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+@pytest.mark.parametrize("lds", [1, 0])
+@pytest.mark.parametrize("cap", [0, 1 << 32])
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 1, 0.7, 7),
+                                                (True, 0, 0.3, 64)])
+def test_jet_block_state(ctx, name, lds, cap, ultra, level, skew, k):
+    """The single-workgroup form with its per-vertex state in LDS or in device
+    memory; a resolve capacity of 0 stalls every rebalance (the state goes
+    back to device memory for the host's global path, then the next launch
+    copies it in again)."""
+    ctx.set_option(heipa.OPT_BLOCK_JET, 1 << 20)
+    ctx.set_option(heipa.OPT_BLOCK_LDS, lds)
+    ctx.set_option(heipa.OPT_RESOLVE_CAP, cap)
+    try:
+        test_jet(ctx, (name, SMALL[name]()), ultra, level, skew, k)
+    finally:
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
+        ctx.set_option(heipa.OPT_BLOCK_LDS, 1)
+        ctx.set_option(heipa.OPT_RESOLVE_CAP, 1 << 32)
 
 
 @pytest.mark.parametrize("copies", [2, 32])
@@ -164,7 +189,7 @@ def test_jet_bucket_copies(ctx, graph, copies, ultra, level, skew, k):
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
         ctx.set_option(heipa.OPT_BUCKET_COPIES, 0)
-        ctx.set_option(heipa.OPT_BLOCK_JET, 1024)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
 
 
 @pytest.mark.parametrize("k,ultra", [(4, False), (16, True), (32, False), (3, False), (7, True)])
