@@ -78,32 +78,9 @@ int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out) {
     heipa_ctx *ctx = new (std::nothrow) heipa_ctx();
     if (!ctx) return HEIPA_ERR_OOM;
     try {
-        Ctx &c = ctx->c;
-        c.device = device;
-        HIP_CHECK(hipSetDevice(device));
-        hipDeviceProp_t prop;
-        HIP_CHECK(hipGetDeviceProperties(&prop, device));
-        c.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-        {
-            int lds = 0;
-            if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
-                c.max_lds = (size_t) lds;
-        }
-        HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-        if (arena_bytes == 0) {
-            size_t free_b = 0, total_b = 0;
-            HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-            arena_bytes = (uint64_t) (free_b / 4);
-        }
-        arena_bytes = (arena_bytes + 255) & ~(uint64_t) 255;
-        HIP_CHECK(hipMalloc(&c.arena_mem, arena_bytes));
-        c.arena.init(c.arena_mem, arena_bytes);
-        c.pinned_bytes = 1 << 18;
-        HIP_CHECK(hipHostMalloc(&c.pinned, c.pinned_bytes, hipHostMallocDefault));
+        ctx_open(ctx->c, device, arena_bytes);
     } catch (const Error &e) {
         std::fprintf(stderr, "heipa_ctx_create: %s\n", e.what());
-        if (ctx->c.arena_mem) (void) hipFree(ctx->c.arena_mem);
-        if (ctx->c.stream) (void) hipStreamDestroy(ctx->c.stream);
         delete ctx;
         return e.code == HEIPA_ERR_DEVICE ? HEIPA_ERR_DEVICE : e.code;
     }
@@ -113,17 +90,7 @@ int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out) {
 
 int heipa_ctx_destroy(heipa_ctx *ctx) {
     if (!ctx) return HEIPA_ERR_INVALID;
-    Ctx &c = ctx->c;
-    (void) hipSetDevice(c.device);
-    if (c.stream) (void) hipStreamSynchronize(c.stream);
-    for (auto &p: c.pending) {
-        (void) hipEventDestroy(p.a);
-        (void) hipEventDestroy(p.b);
-    }
-    for (auto e: c.event_pool) (void) hipEventDestroy(e);
-    if (c.pinned) (void) hipHostFree(c.pinned);
-    if (c.arena_mem) (void) hipFree(c.arena_mem);
-    if (c.stream) (void) hipStreamDestroy(c.stream);
+    ctx_close(ctx->c);
     delete ctx;
     return HEIPA_OK;
 }
@@ -438,6 +405,7 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         else if (option == HEIPA_OPT_PROF_SAMPLE) c.prof_sample = value < 1 ? 1u : (u32) std::min<uint64_t>(value, 1u << 30);
         else if (option == HEIPA_OPT_BUCKET_COPIES) c.bucket_copies = (u32) std::min<uint64_t>(value, 32);
         else if (option == HEIPA_OPT_BLOCK_LDS) c.block_lds = value != 0;
+        else if (option == HEIPA_OPT_HM_WORKERS) c.hm_workers = (u32) std::min<uint64_t>(value, 64);
         else require(false, "unknown option");
     });
 }

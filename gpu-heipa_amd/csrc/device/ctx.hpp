@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -157,6 +158,7 @@ struct Ctx {
     bool use_graphs = true;  // HEIPA_OPT_GRAPHS
     u32 bucket_copies = 0;   // HEIPA_OPT_BUCKET_COPIES (0: automatic)
     bool block_lds = true;   // HEIPA_OPT_BLOCK_LDS
+    u32 hm_workers = 8;      // HEIPA_OPT_HM_WORKERS
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;
@@ -177,6 +179,9 @@ struct Ctx {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
+    // worker contexts (own stream, arena and staging) for independent
+    // subproblems run concurrently, kept between calls
+    std::vector<std::unique_ptr<Ctx>> workers;
 
     hipEvent_t get_event() {
         if (!event_pool.empty()) {
@@ -217,6 +222,89 @@ struct Ctx {
     template<typename T>
     T *pinned_as() { return (T *) pinned; }
 };
+
+// Opens a context on `device`: a non-blocking stream, an arena of
+// arena_bytes (0: a quarter of the free device memory) and pinned staging.
+// On failure everything opened so far is released and the error rethrown.
+inline void ctx_close(Ctx &c);
+inline void ctx_open(Ctx &c, int device, uint64_t arena_bytes) {
+    try {
+        c.device = device;
+        HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        c.num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+            c.max_lds = (size_t) lds;
+        HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        if (arena_bytes == 0) {
+            size_t free_b = 0, total_b = 0;
+            HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+            arena_bytes = (uint64_t) (free_b / 4);
+        }
+        arena_bytes = (arena_bytes + 255) & ~(uint64_t) 255;
+        HIP_CHECK(hipMalloc(&c.arena_mem, arena_bytes));
+        c.arena.init(c.arena_mem, arena_bytes);
+        c.pinned_bytes = 1 << 18;
+        HIP_CHECK(hipHostMalloc(&c.pinned, c.pinned_bytes, hipHostMallocDefault));
+    } catch (...) {
+        ctx_close(c);
+        throw;
+    }
+}
+
+// Releases a context and its workers (errors ignored: teardown).
+inline void ctx_close(Ctx &c) {
+    for (auto &w: c.workers)
+        if (w) ctx_close(*w);
+    c.workers.clear();
+    (void) hipSetDevice(c.device);
+    if (c.stream) (void) hipStreamSynchronize(c.stream);
+    for (auto &p: c.pending) {
+        (void) hipEventDestroy(p.a);
+        (void) hipEventDestroy(p.b);
+    }
+    c.pending.clear();
+    for (auto e: c.event_pool) (void) hipEventDestroy(e);
+    c.event_pool.clear();
+    if (c.pinned) (void) hipHostFree(c.pinned);
+    if (c.arena_mem) (void) hipFree(c.arena_mem);
+    if (c.stream) (void) hipStreamDestroy(c.stream);
+    c.pinned = nullptr;
+    c.arena_mem = nullptr;
+    c.stream = nullptr;
+}
+
+// The tuning options of `from` (results never depend on them) and its
+// profiling settings, for a worker that runs part of its call.
+inline void ctx_copy_settings(Ctx &to, const Ctx &from) {
+    to.resolve_cap = from.resolve_cap;
+    to.block_jet_max_n = from.block_jet_max_n;
+    to.use_graphs = from.use_graphs;
+    to.bucket_copies = from.bucket_copies;
+    to.block_lds = from.block_lds;
+    to.hm_workers = from.hm_workers;
+    to.prof_on = from.prof_on;
+    to.prof_filter = from.prof_filter;
+    to.prof_sample = from.prof_sample;
+}
+
+// Adds a worker's kernel statistics to `to` and clears them in the worker
+// (after the worker's stream was synchronised).
+inline void ctx_merge_prof(Ctx &to, Ctx &w) {
+    for (size_t i = 0; i < w.prof_names.size(); ++i) {
+        const KernelStat &s = w.prof_stats[i];
+        KernelStat &t = to.prof_stats[to.prof_slot(w.prof_names[i].c_str())];
+        t.launches += s.launches;
+        t.timed += s.timed;
+        t.ms += s.ms;
+        t.bytes += s.bytes;
+    }
+    w.prof_names.clear();
+    w.prof_index.clear();
+    w.prof_stats.clear();
+}
 
 // Adds data-dependent algorithmic bytes to a timed kernel's total (for kernels
 // whose compulsory traffic is only known after they ran).

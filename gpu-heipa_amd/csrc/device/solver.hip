@@ -21,6 +21,9 @@
 
 #include <chrono>
 #include <cmath>
+#include <exception>
+#include <system_error>
+#include <thread>
 
 namespace heipa {
 namespace {
@@ -183,6 +186,154 @@ struct HMParams {
     bool ultra;
 };
 
+// Builds child `id` of g (the vertices with tmp == id, their internal edges
+// in adjacency order) on the front stack of c's arena; false: the child is
+// empty (nothing pushed).
+bool build_child(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_o, u32 id, DGraph &child,
+                 u32 *&c_n_to_o) {
+    u32 cn = 0, cm = 0;
+    s64 cw = 0;
+    u32 *local_ids, *cxadj, *cadj;
+    s32 *c_vw, *cew;
+    {
+        BackScope bs(c.arena);
+        u32 *loc = bs.get<u32>(g.n);
+        u32 *scal = bs.get<u32>(4);
+        // child size (compaction count) first
+        scan<u32>(c, "hm_child_index", ChildSrc{tmp, id}, LocalIdDst{tmp, id, loc}, g.n, scal);
+        readback(c, &cn, scal, 1);
+        if (cn == 0) return false;
+        c_vw = (s32 *) c.arena.push_front(4ull * cn);
+        cxadj = (u32 *) c.arena.push_front(4ull * (cn + 1));
+        c_n_to_o = (u32 *) c.arena.push_front(4ull * cn);
+        local_ids = bs.get<u32>(cn);
+        copy_d2d(c, local_ids, loc, 4ull * cn);
+        scan<u32>(c, "hm_child_index", ChildSrc{tmp, id},
+                  ChildDst{tmp, id, loc, c_n_to_o, n_to_o, c_vw, g.vw}, g.n, scal);
+        auto *acc = (unsigned long long *) bs.get<unsigned long long>(1);
+        fill<unsigned long long>(c, acc, 1, 0ull);
+        launch(c, "hm_child_weight", 4.0 * cn, dim3(grid_for(c, cn, 256)), dim3(256), 0, child_weight_kernel,
+               (const s32 *) c_vw, cn, acc);
+        u32 *cdeg = bs.get<u32>(cn + 1);
+        launch(c, "hm_child_degree", 0.0, dim3(grid_for(c, cn, 256)), dim3(256), 0, child_degree_kernel, g,
+               (const u32 *) tmp, id, (const u32 *) local_ids, cn, cdeg);
+        exclusive_sum<u32>(c, "hm_child_degree", cdeg, cxadj, cn);
+        unsigned long long wsum;
+        readback(c, &wsum, acc, 1);
+        cw = (s64) wsum;
+        readback(c, &cm, cxadj + cn, 1);
+        cadj = (u32 *) c.arena.push_front(4ull * (cm ? cm : 1));
+        cew = (s32 *) c.arena.push_front(4ull * (cm ? cm : 1));
+        launch(c, "hm_child_fill", 0.0, dim3(grid_for(c, cn, 4, 8)), dim3(256), 0, child_fill_kernel, g,
+               (const u32 *) tmp, id, (const u32 *) loc, (const u32 *) local_ids, cn, (const u32 *) cxadj, cadj,
+               cew);
+        // n_to_o of the child: global ids (ChildDst wrote n_to_o[u])
+    }
+    child.n = cn;
+    child.m = cm;
+    child.gw = cw;
+    child.xadj = cxadj;
+    child.adj = cadj;
+    child.ew = cew;
+    child.vw = c_vw;
+    return true;
+}
+
+void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u32> &ident, const HMParams &p,
+            u32 *global);
+
+// Arena bytes of one child subtree's solve (the pool's job estimate for a
+// multisection of what remains of the hierarchy).
+u64 hm_child_bytes(const DGraph &g, const HMParams &p, u32 level) {
+    u64 k = 1;
+    for (u32 i = 0; i <= level; ++i) k = std::max<u64>(k, p.hier[i]);
+    const u64 n = g.n, m = g.m;
+    const u64 sections = std::max<u64>(1, std::min<u64>(128, n / (4096 * k) + 1));
+    return 272 * n + 72 * m + 50ull * k * sections * 32 + (64ull << 20);
+}
+
+// The subtrees below the first split are independent (disjoint vertices,
+// disjoint entries of the output, deterministic solves), so they run at the
+// same time on worker contexts, one host thread each; the result does not
+// depend on their order.  The children are built here first, on c's
+// stream; a worker context that cannot be opened leaves the caller on the
+// sequential path (false, nothing changed but the children built).
+bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_o, u32 level, const HMParams &p,
+                       u32 *global) {
+    struct Kid {
+        DGraph g;
+        u32 *n_to_o;
+        u32 id;
+    };
+    std::vector<Kid> kids;
+    const u32 k = p.hier[level];
+    for (u32 id = 0; id < k; ++id) {
+        Kid kd{};
+        kd.id = id;
+        if (build_child(c, g, tmp, n_to_o, id, kd.g, kd.n_to_o)) kids.push_back(kd);
+    }
+    if (kids.size() < 2) {
+        for (const Kid &kd: kids) {
+            std::vector<u32> ident{kd.id};
+            hm_rec(c, kd.g, kd.n_to_o, level - 1, ident, p, global);
+        }
+        return true;
+    }
+    c.sync();  // the children are complete before other streams read them
+    const size_t nw = std::min<size_t>(kids.size(), c.hm_workers);
+    u64 need = 0;
+    for (const Kid &kd: kids) need = std::max(need, hm_child_bytes(kd.g, p, level - 1));
+    try {
+        if (c.workers.size() < nw) c.workers.resize(nw);
+        for (size_t i = 0; i < nw; ++i) {
+            auto &w = c.workers[i];
+            if (w && w->arena.capacity() >= need) continue;
+            if (w) ctx_close(*w);
+            w = std::make_unique<Ctx>();
+            ctx_open(*w, c.device, need);
+        }
+    } catch (const Error &) {
+        for (auto &w: c.workers)
+            if (w) ctx_close(*w);
+        c.workers.clear();
+        for (const Kid &kd: kids) {
+            std::vector<u32> ident{kd.id};
+            hm_rec(c, kd.g, kd.n_to_o, level - 1, ident, p, global);
+        }
+        return true;
+    }
+    // workers take the children in turn: kid i runs on worker i mod nw,
+    // each worker's children one after another on its own stream
+    std::vector<std::exception_ptr> errors(nw);
+    auto run = [&](size_t wi) {
+        try {
+            Ctx &w = *c.workers[wi];
+            HIP_CHECK(hipSetDevice(w.device));
+            ctx_copy_settings(w, c);
+            for (size_t i = wi; i < kids.size(); i += nw) {
+                std::vector<u32> ident{kids[i].id};
+                hm_rec(w, kids[i].g, kids[i].n_to_o, level - 1, ident, p, global);
+            }
+            w.sync();
+        } catch (...) {
+            errors[wi] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> threads;
+    for (size_t wi = 0; wi < nw; ++wi) {
+        try {
+            threads.emplace_back(run, wi);
+        } catch (const std::system_error &) {
+            run(wi);  // no thread available: this worker's children run here
+        }
+    }
+    for (auto &t: threads) t.join();
+    for (size_t wi = 0; wi < nw; ++wi) ctx_merge_prof(c, *c.workers[wi]);
+    for (auto &e: errors)
+        if (e) std::rethrow_exception(e);
+    return true;
+}
+
 void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u32> &ident, const HMParams &p,
             u32 *global) {
     const u32 l = (u32) p.hier.size();
@@ -200,54 +351,15 @@ void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u
         c.arena.release_front(depth0);
         return;
     }
+    if (ident.empty() && level > 0 && c.hm_workers > 1 && hm_split_parallel(c, g, tmp, n_to_o, level, p, global)) {
+        c.arena.release_front(depth0);
+        return;
+    }
     for (u32 id = 0; id < k; ++id) {
         const size_t cdepth = c.arena.front_depth();
-        u32 cn = 0, cm = 0;
-        s64 cw = 0;
-        u32 *local_ids, *c_n_to_o, *cxadj, *cadj;
-        s32 *c_vw, *cew;
-        {
-            BackScope bs(c.arena);
-            u32 *loc = bs.get<u32>(g.n);
-            u32 *scal = bs.get<u32>(4);
-            // child size (compaction count) first
-            scan<u32>(c, "hm_child_index", ChildSrc{tmp, id}, LocalIdDst{tmp, id, loc}, g.n, scal);
-            readback(c, &cn, scal, 1);
-            if (cn == 0) continue;
-            c_vw = (s32 *) c.arena.push_front(4ull * cn);
-            cxadj = (u32 *) c.arena.push_front(4ull * (cn + 1));
-            c_n_to_o = (u32 *) c.arena.push_front(4ull * cn);
-            local_ids = bs.get<u32>(cn);
-            copy_d2d(c, local_ids, loc, 4ull * cn);
-            scan<u32>(c, "hm_child_index", ChildSrc{tmp, id},
-                      ChildDst{tmp, id, loc, c_n_to_o, n_to_o, c_vw, g.vw}, g.n, scal);
-            auto *acc = (unsigned long long *) bs.get<unsigned long long>(1);
-            fill<unsigned long long>(c, acc, 1, 0ull);
-            launch(c, "hm_child_weight", 4.0 * cn, dim3(grid_for(c, cn, 256)), dim3(256), 0, child_weight_kernel,
-                   (const s32 *) c_vw, cn, acc);
-            u32 *cdeg = bs.get<u32>(cn + 1);
-            launch(c, "hm_child_degree", 0.0, dim3(grid_for(c, cn, 256)), dim3(256), 0, child_degree_kernel, g,
-                   (const u32 *) tmp, id, (const u32 *) local_ids, cn, cdeg);
-            exclusive_sum<u32>(c, "hm_child_degree", cdeg, cxadj, cn);
-            unsigned long long wsum;
-            readback(c, &wsum, acc, 1);
-            cw = (s64) wsum;
-            readback(c, &cm, cxadj + cn, 1);
-            cadj = (u32 *) c.arena.push_front(4ull * (cm ? cm : 1));
-            cew = (s32 *) c.arena.push_front(4ull * (cm ? cm : 1));
-            launch(c, "hm_child_fill", 0.0, dim3(grid_for(c, cn, 4, 8)), dim3(256), 0, child_fill_kernel, g,
-                   (const u32 *) tmp, id, (const u32 *) loc, (const u32 *) local_ids, cn, (const u32 *) cxadj, cadj,
-                   cew);
-            // n_to_o of the child: global ids (ChildDst wrote n_to_o[u])
-        }
         DGraph child;
-        child.n = cn;
-        child.m = cm;
-        child.gw = cw;
-        child.xadj = cxadj;
-        child.adj = cadj;
-        child.ew = cew;
-        child.vw = c_vw;
+        u32 *c_n_to_o = nullptr;
+        if (!build_child(c, g, tmp, n_to_o, id, child, c_n_to_o)) continue;
         ident.push_back(id);
         hm_rec(c, child, c_n_to_o, level - 1, ident, p, global);
         ident.pop_back();

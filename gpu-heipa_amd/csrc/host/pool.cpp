@@ -132,6 +132,9 @@ int heipa_pool_create(int device, uint32_t slots, uint64_t arena_bytes, uint64_t
             delete p;
             return rc;
         }
+        // the pool's slots already overlap whole jobs: an HM job solves its
+        // subtrees on its own slot, one after another
+        heipa_ctx_set_option(c, HEIPA_OPT_HM_WORKERS, 1);
         p->ctxs.push_back(c);
     }
     try {

@@ -26,6 +26,7 @@ OPT_GRAPHS = 3
 OPT_PROF_SAMPLE = 4
 OPT_BUCKET_COPIES = 5
 OPT_BLOCK_LDS = 6
+OPT_HM_WORKERS = 7
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -236,6 +236,11 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
  * for the whole launch when it fits (copied in at the start, back at the end); 0 keeps it in device
  * memory.  Results do not depend on it. */
 #define HEIPA_OPT_BLOCK_LDS 6
+/* HEIPA_OPT_HM_WORKERS: worker contexts (each its own stream, arena and host thread, kept by the
+ * context between calls) on which heipa_hm_partition solves the subtrees below its first split at
+ * the same time (default 8, at most 64; 0 or 1: one after another on the context's stream).  Results
+ * do not depend on it. */
+#define HEIPA_OPT_HM_WORKERS 7
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */

@@ -217,3 +217,15 @@ def test_hm(ctx, hier):
     D = orc.distance_matrix(hier, [1, 10, 100])
     Dd = heipa.from_numpy_u32(D.reshape(-1).view(np.uint32))
     assert ctx.comm_cost(dg, heipa.from_numpy_u32(gp), Dd, k) == orc.comm_cost(g, op, D)
+
+
+@pytest.mark.parametrize("workers", [0, 2])
+def test_hm_workers(ctx, workers):
+    """The subtrees below the first split solved one after another (0) or on
+    two worker contexts for six children (in turns) give the same mapping."""
+    ctx.set_option(heipa.OPT_HM_WORKERS, workers)
+    try:
+        test_hm(ctx, [4, 8, 6])
+        test_hm(ctx, [3, 1, 2])
+    finally:
+        ctx.set_option(heipa.OPT_HM_WORKERS, 8)
