@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device (the known-size kernel that scales HBM counters)")
     ap.add_argument("--slots", type=int, default=8, help="batched/served: pool workers")
+    ap.add_argument("--hm-workers", type=int, default=None,
+                    help="HM: worker contexts for the subtrees below the first split (default: the library's)")
     ap.add_argument("--block-jet-n", type=int, default=None,
                     help="largest level run by the single-workgroup Jet (default: the library's)")
     ap.add_argument("--batch-count", type=int, default=4096, help="batched: number of graphs")
@@ -91,6 +93,8 @@ def main():
     ctx = heipa.Context(0, 16 << 30)
     if args.block_jet_n is not None:
         ctx.set_option(heipa.OPT_BLOCK_JET, args.block_jet_n)
+    if args.hm_workers is not None:
+        ctx.set_option(heipa.OPT_HM_WORKERS, args.hm_workers)
     dg = heipa.DeviceGraph(g, "cuda:0")
     part = torch.empty(g.n, dtype=torch.int32, device="cuda:0")
     k, ultra = w["k"], w["ultra"]
