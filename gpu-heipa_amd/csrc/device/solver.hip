@@ -306,8 +306,9 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
     // each worker's children one after another on its own stream
     std::vector<std::exception_ptr> errors(nw);
     auto run = [&](size_t wi) {
+        Ctx &w = *c.workers[wi];
+        const size_t fd = w.arena.front_depth(), bd = w.arena.back_depth();
         try {
-            Ctx &w = *c.workers[wi];
             HIP_CHECK(hipSetDevice(w.device));
             ctx_copy_settings(w, c);
             for (size_t i = wi; i < kids.size(); i += nw) {
@@ -317,6 +318,10 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
             w.sync();
         } catch (...) {
             errors[wi] = std::current_exception();
+            // the worker is reused by later calls: leave its arena as it was
+            (void) hipStreamSynchronize(w.stream);
+            w.arena.release_back(bd);
+            w.arena.release_front(fd);
         }
     };
     std::vector<std::thread> threads;
