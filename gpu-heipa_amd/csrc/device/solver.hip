@@ -292,7 +292,7 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
             w = std::make_unique<Ctx>();
             ctx_open(*w, c.device, need);
         }
-    } catch (const Error &) {
+    } catch (const std::exception &) {  // device or host memory for the workers: one after another here
         for (auto &w: c.workers)
             if (w) ctx_close(*w);
         c.workers.clear();
