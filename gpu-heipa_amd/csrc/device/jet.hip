@@ -1276,10 +1276,10 @@ __device__ __forceinline__ void sort_keys_one_per_thread(const u64 *src, u32 L, 
     __syncthreads();
 }
 
-// Resolves a list of L entries with RB threads; false: the list (or the
+// Resolves a list of L entries with NT threads; false: the list (or the
 // strong overflow) does not fit and nothing was appended.  Moves go to the
 // extra move list, counted in LDS (`local_cnt`) or in st[ST_MOVES].
-template<u32 CAP>
+template<u32 CAP, int NT = RB>
 __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, const ResolveLds &lds, u32 *local_cnt) {
     u64 *skey = lds.skey;
     s64 *soff = lds.soff;
@@ -1294,16 +1294,16 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
     while (P2 < L) P2 <<= 1;
     if (threadIdx.x == 0) ocount = 0;
     if (strong)
-        for (u32 b = threadIdx.x; b < d.k; b += RB) d.evict_adjust[b] = 0;
-    if (P2 <= RB) {
+        for (u32 b = threadIdx.x; b < d.k; b += NT) d.evict_adjust[b] = 0;
+    if (P2 <= NT) {
         sort_keys_one_per_thread(d.rlist, L, P2, skey);
     } else {
-    for (u32 i = threadIdx.x; i < P2; i += RB) skey[i] = i < L ? d.rlist[i] : ~0ull;
+    for (u32 i = threadIdx.x; i < P2; i += NT) skey[i] = i < L ? d.rlist[i] : ~0ull;
     __syncthreads();
     // bitonic sort, ascending
     for (u32 kk = 2; kk <= P2; kk <<= 1) {
         for (u32 j = kk >> 1; j > 0; j >>= 1) {
-            for (u32 i = threadIdx.x; i < P2; i += RB) {
+            for (u32 i = threadIdx.x; i < P2; i += NT) {
                 const u32 p = i ^ j;
                 if (p > i) {
                     const u64 a = skey[i], b = skey[p];
@@ -1319,20 +1319,20 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
     }
     // exclusive prefix of the weights in sorted order
     s64 carry = 0;
-    for (u32 i0 = 0; i0 < L; i0 += RB) {
+    for (u32 i0 = 0; i0 < L; i0 += NT) {
         const u32 i = i0 + threadIdx.x;
         const s64 w = i < L ? vwt(d, (u32) skey[i]) : 0;
         s64 tot;
-        const s64 e = block_exclusive_scan<RB>(w, red, tot);
+        const s64 e = block_exclusive_scan<NT>(w, red, tot);
         if (i < L) soff[i] = carry + e;
         carry += tot;
     }
     __syncthreads();
     // offset inside the minibucket = prefix - prefix at the minibucket's first entry
-    s64 offv[CAP / RB];
+    s64 offv[CAP / NT];
 #pragma unroll
-    for (u32 q = 0; q < CAP / RB; ++q) {
-        const u32 i = q * RB + threadIdx.x;
+    for (u32 q = 0; q < CAP / NT; ++q) {
+        const u32 i = q * NT + threadIdx.x;
         offv[q] = 0;
         if (i < L) {
             const u64 head = skey[i] & 0xFFFFFFFF00000000ull;
@@ -1347,12 +1347,12 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
     }
     __syncthreads();
 #pragma unroll
-    for (u32 q = 0; q < CAP / RB; ++q) {
-        const u32 i = q * RB + threadIdx.x;
+    for (u32 q = 0; q < CAP / NT; ++q) {
+        const u32 i = q * NT + threadIdx.x;
         if (i < L) soff[i] = offv[q];
     }
     __syncthreads();
-    for (u32 i = threadIdx.x; i < L; i += RB) {
+    for (u32 i = threadIdx.x; i < L; i += NT) {
         const u32 u = (u32) skey[i];
         const u32 g = (u32) (skey[i] >> 32);
         const u32 ub = d.work[u];
@@ -1372,11 +1372,11 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
         __syncthreads();
         const bool es_lds = d.k + 1 <= ES_CAP;
         if (es_lds)
-            for (u32 i = threadIdx.x; i <= d.k; i += RB) s_es[i] = d.evict_start[i];
+            for (u32 i = threadIdx.x; i <= d.k; i += NT) s_es[i] = d.evict_start[i];
         __syncthreads();
         const s64 *es = es_lds ? s_es : d.evict_start;
         // adjust_scores, first pass; overflowing vertices are resolved below
-        for (u32 i = threadIdx.x; i < L; i += RB) {
+        for (u32 i = threadIdx.x; i < L; i += NT) {
             if (!ssel[i]) continue;
             const u32 u = (u32) skey[i];
             const u32 g = (u32) (skey[i] >> 32);
@@ -1420,7 +1420,7 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
         }
         __syncthreads();
     }
-    for (u32 i0 = 0; i0 < L; i0 += RB) {
+    for (u32 i0 = 0; i0 < L; i0 += NT) {
         const u32 i = i0 + threadIdx.x;
         const bool take = i < L && ssel[i] != 0;
         if (local_cnt) append_local(take, take ? (u32) skey[i] : 0u, d.moves_x, local_cnt);
@@ -1968,7 +1968,7 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
 // the loop: the iteration's lengths and statistics are published in st[] and
 // region counts are zero, so the host's global path and the multi-launch
 // apply and control finish that iteration.
-constexpr int BM = 1024;
+constexpr int BM = 512;  // 256 registers per lane (1024 threads cap them at 128 and spill)
 constexpr u32 BM_PIECE = 2048;
 constexpr u32 BM_RCAP = 2048;
 constexpr size_t BM_CONN_LDS = 32 * 1024;
@@ -2202,7 +2202,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
         // ---- resolve
         const u32 L = s_nrl;
         if (mode) {
-            const bool ok = resolve_body<BM_RCAP>(d, strong, stamp, L, rl, &s_nmoves);
+            const bool ok = resolve_body<BM_RCAP, BM>(d, strong, stamp, L, rl, &s_nmoves);
             if (!ok) {
                 if (tid == 0) {
                     ctl->st[ST_MISS] = s_acc[PT_MISS];
