@@ -1971,7 +1971,7 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
 constexpr int BM = 512;  // 256 registers per lane (1024 threads cap them at 128 and spill)
 constexpr u32 BM_PIECE = 2048;
 constexpr u32 BM_RCAP = 2048;
-constexpr size_t BM_CONN_LDS = 32 * 1024;
+constexpr size_t BM_CONN_LDS = 40 * 1024;
 
 __host__ __device__ inline size_t bm_sweep_lds_bytes(u32 k, u32 groups) {
     return conn_group_bytes(k) * groups + 4ull * BM_PIECE + 16 + 4ull * k;
