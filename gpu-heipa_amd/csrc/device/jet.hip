@@ -176,6 +176,13 @@ struct JetDev {
     const s32 *D;       // k x k distances: mapping objective J (ProMap); null: edge cut
 };
 
+// Per-vertex staleness of the caches, the n bytes after the locks: bit 0
+// the decision cache (dcache, gcache), bit 1 the interior cache.  Moving a
+// vertex marks it and its neighbours stale with one byte each instead of
+// rewriting both caches (12 bytes): the bytes stay in L2 where the caches
+// (12n) do not, and most moves of a rebalancing iteration touch vertices
+// many times.
+__device__ __forceinline__ u8 *stale_of(const JetDev &d) { return d.lock + d.n; }
 __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64) d.vw[u] : 1; }
 
 // a candidate's weight into minibucket g: rebalancing puts most candidates of
@@ -455,6 +462,7 @@ __device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u3
         }
         d.gcache[u] = gain;
         d.dcache[u] = best_id;
+        stale_of(d)[u] = 0;  // both caches rewritten
         d.dest[u] = best_id;
         if (best_id != NO_MOVE && !locked) {
             d.gain1[u] = gain;
@@ -532,6 +540,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
         if (active && lane == 0) {
             const s64 own = (s64) gc.cnt[own_b];
             d.icache[u] = (T == 0 || (T == 1 && gc.touched[0] == own_b)) ? own : -1;
+            stale_of(d)[u] = stale_of(d)[u] & 1u;  // the interior cache rewritten
             u32 dst;
             s64 gain;
             if (best.b != NO_MOVE && best.c > 0) {
@@ -566,6 +575,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
         if (active && lane == 0) {
             const s64 own = (s64) gc.cnt[own_b];
             d.icache[u] = (T == 0 || (T == 1 && gc.touched[0] == own_b)) ? own : -1;
+            stale_of(d)[u] = stale_of(d)[u] & 1u;  // the interior cache rewritten
             if (count == 0) count = 1;
             const s64 gain = sum / count - own;
             const s64 wu = vwt(d, u);
@@ -661,8 +671,9 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
             if (vv < p1) {
                 if (save) d.part[v] = d.work[v];
                 if (mode == 0) {
-                    const u32 dc = d.dcache[v];
-                    const u8 lk = d.lock[v];  // loaded with the cache entry: one round trip
+                    const u32 dc0 = d.dcache[v];
+                    const u8 lk = d.lock[v], sv = stale_of(d)[v];  // loaded with the cache entry: one round trip
+                    const u32 dc = (sv & 1u) ? NULL_PART : dc0;
                     if (dc == NULL_PART) {
                         // offsets and own block in one round trip, kept for phase B
                         const u32 xb = d.xadj[v], xe = d.xadj[v + 1], own = d.work[v];
@@ -682,7 +693,8 @@ __global__ __launch_bounds__(JB) void sweep_kernel(JetDev d) {
                 } else {
                     const u32 wv = d.work[v], xb = d.xadj[v], xe = d.xadj[v + 1];
                     if (xe - xb <= d.hub_deg && reb_eligible(d, strong, rs, v, wv)) {
-                        const s64 ic = d.icache[v];
+                        const s64 ic0 = d.icache[v];
+                        const s64 ic = (stale_of(d)[v] & 2u) ? -1 : ic0;
                         if (ic >= 0) {
                             cand = reb_interior(d, strong, rs, v, ic);
                         } else {
@@ -818,7 +830,8 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
     const bool strong = mode == 2;
     s64 mbw = 0;
     if (mode == 0) {
-        if (d.dcache[u] != NULL_PART) return;
+        const u32 dc0 = d.dcache[u];
+        if (!(stale_of(d)[u] & 1u) && dc0 != NULL_PART) return;
     } else {
         mbw = max_block_weight(d, strong);
         reb_prologue(d, strong, mbw, rs);
@@ -1488,8 +1501,7 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
                 const s64 x = w[q] * ((s64) (nu != nv) - (s64) (ou != ov));
                 delta += mv ? x : 2 * x;
             }
-            d.dcache[v[q]] = NULL_PART;
-            d.icache[v[q]] = -1;
+            stale_of(d)[v[q]] = 3;
         }
     }
     return delta;
@@ -1505,8 +1517,7 @@ __device__ __forceinline__ s64 move_delta(const JetDev &d, u32 u, u32 lane, u32 
 __device__ __forceinline__ void move_commit(const JetDev &d, u32 u, s64 *bw) {
     const u32 ou = d.oldp[u], nu = d.dest[u];
     const s64 w = vwt(d, u);
-    d.dcache[u] = NULL_PART;
-    d.icache[u] = -1;
+    stale_of(d)[u] = 3;
     d.work[u] = nu;
     atomicAdd((unsigned long long *) &bw[ou], (unsigned long long) (-w));
     atomicAdd((unsigned long long *) &bw[nu], (unsigned long long) w);
@@ -1889,8 +1900,7 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
                     const s64 x = w[q] * ((s64) (nu != nv) - (s64) (ou != ov));
                     acc += mv ? x : 2 * x;
                 }
-                d.dcache[v[q]] = NULL_PART;
-                d.icache[v[q]] = -1;
+                stale_of(d)[v[q]] = 3;
             }
         }
         if (t < cnt && !hub) {
@@ -1979,8 +1989,8 @@ __host__ __device__ inline size_t bm_sweep_lds_bytes(u32 k, u32 groups) {
 
 // LDS copy of the per-vertex state of the single-workgroup form: s64
 // gcache, gain1, icache (n each) and the k block weights, then u32 work,
-// dest, dcache, mark, oldp, gid (n each), then the u8 locks
-__host__ __device__ constexpr size_t bm_state_bytes(u32 n, u32 k) { return 49ull * n + 8ull * k; }
+// dest, dcache, mark, oldp, gid (n each), then the u8 locks and stale bytes
+__host__ __device__ constexpr size_t bm_state_bytes(u32 n, u32 k) { return 50ull * n + 8ull * k; }
 
 // LS: the per-vertex state lives in LDS (from byte state_off of the dynamic
 // LDS) for the whole launch: copied in at the start and back to device
@@ -2017,6 +2027,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             d.oldp[v] = dg.oldp[v];
             d.gid[v] = dg.gid[v];
             d.lock[v] = dg.lock[v];
+            d.lock[n + v] = dg.lock[n + v];  // stale bytes
         }
         for (u32 b = threadIdx.x; b < k; b += BM) d.bw[b] = dg.bw[b];
         __syncthreads();
@@ -2075,7 +2086,8 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
                 if (vv < p1) {
                     if (save) d.part[v] = d.work[v];
                     if (mode == 0) {
-                        const u32 dc = d.dcache[v];
+                        const u32 dc0 = d.dcache[v];
+                        const u32 dc = (stale_of(d)[v] & 1u) ? NULL_PART : dc0;
                         if (dc == NULL_PART) {
                             heavy = true;
                         } else {
@@ -2089,7 +2101,8 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
                             }
                         }
                     } else if (reb_eligible(d, strong, rs, v, d.work[v])) {
-                        const s64 ic = d.icache[v];
+                        const s64 ic0 = d.icache[v];
+                        const s64 ic = (stale_of(d)[v] & 2u) ? -1 : ic0;
                         if (ic >= 0) cand = reb_interior(d, strong, rs, v, ic);
                         else heavy = true;
                     }
@@ -2262,6 +2275,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             dg.oldp[v] = d.oldp[v];
             dg.gid[v] = d.gid[v];
             dg.lock[v] = d.lock[v];
+            dg.lock[dg.n + v] = d.lock[dg.n + v];
         }
         for (u32 b = threadIdx.x; b < dg.k; b += BM) dg.bw[b] = d.bw[b];
     }
@@ -2540,7 +2554,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.gcache = bs.get<s64>(n);
     d.icache = bs.get<s64>(n);
     d.gain1 = bs.get<s64>(n);
-    d.lock = bs.get<u8>(n);
+    d.lock = bs.get<u8>(2ull * n);  // locks, then the stale bytes
     d.bw = bs.get<s64>(k);
     // minibuckets (jet:364-371, :605-612)
     {
@@ -2670,7 +2684,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     copy_d2d(c, d.work, part, 4ull * n);
     fill<u32>(c, d.dcache, n, NULL_PART);
     fill<s64>(c, d.icache, n, -1);
-    fill<u8>(c, d.lock, n, (u8) 0);
+    fill<u8>(c, d.lock, 2ull * n, (u8) 0);
     fill<u32>(c, d.mark, n, 0u);
     fill<u32>(c, h.over_flag, n, 0u);
     fill<s64>(c, d.btot, (u64) d.nshard * ((u64) d.t + 1), 0);
