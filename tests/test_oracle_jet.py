@@ -1,0 +1,372 @@
+"""CPU pin of the oracle's plain Jet label propagation (jet_lp,
+src/refinement/jet_label_propagation.h:182-354) by brute force.
+
+The reference ships no tests or fixtures for Jet, so its first LP iteration
+is recomputed here from whole-graph edge-cut evaluations alone:
+  * a vertex's connectivity to its own block is the cut increase when it
+    moves to a block nobody else is in;
+  * the gain of moving it to an adjacent block b is the cut drop of that lone
+    move, and the destination is the adjacent block with the largest drop
+    (the smaller id on ties, jet:219-221);
+  * a move with negative gain stays a candidate only while the loss is below
+    floor(ratio * own connectivity) (jet:227-228); the ratio is 0.25 at the
+    finest level, 0.75 above it and 0.85 first in the ultra schedule
+    (jet:953-961);
+  * the afterburner (jet:269-328) keeps a candidate when its gain plus the
+    change of that gain once the candidates ordered before it (larger gain,
+    then smaller id) have moved is not negative; the change is recomputed
+    from cut evaluations of the graph with and without those moves.
+The oracle's trace of that iteration must agree on the candidate count, the
+gain hash, the move count, the move hash and the cut after the moves.
+
+The first weak rebalance iteration (rebalance_weak, jet:592-751) is pinned the
+same way: connectivities and gains come from cut evaluations, the gain bucket
+from the exact-arithmetic restatement (itself pinned in test_oracle.py), and
+the selection is stated as a sort, not as the bucket prefix sums the oracle
+and the reference use: in every overloaded block the candidates are ordered by
+(gain bucket, id mod sections, id) and a candidate moves while the weight of
+the candidates before it is below the block's excess over lmax.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import heipa
+import heipa_oracle as orc
+import refsim
+from graphs import block_weights, random_partition
+
+U64 = (1 << 64) - 1
+
+
+def _cut(g, part):
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj.astype(np.int64)))
+    w = np.ones(len(g.adjncy), np.int64) if g.adjwgt is None else g.adjwgt.astype(np.int64)
+    tot = int((w * (part[src] != part[g.adjncy.astype(np.int64)])).sum())
+    assert tot % 2 == 0
+    return tot // 2
+
+
+def _moved(part, moves):
+    p = part.copy()
+    for u, b in moves.items():
+        p[u] = b
+    return p
+
+
+def _first_lp_iteration(g, part, k, ratio):
+    c0 = _cut(g, part)
+    gain, dest = {}, {}
+    for u in range(g.n):
+        own_b = int(part[u])
+        own = _cut(g, _moved(part, {u: k})) - c0  # block k holds no vertex
+        blocks = sorted({int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]} - {own_b})
+        best, best_b = None, None
+        for b in blocks:
+            drop = c0 - _cut(g, _moved(part, {u: b}))
+            if best is None or drop > best:
+                best, best_b = drop, b
+        if best_b is None:
+            continue
+        if best >= 0 or -best < math.floor(ratio * own):
+            gain[u], dest[u] = best, best_b
+    moves = []
+    for u in sorted(gain):
+        first = {int(v): dest[int(v)] for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]
+                 if int(v) in gain and (gain[int(v)] > gain[u] or (gain[int(v)] == gain[u] and v < u))}
+        p1 = _moved(part, first)
+        after = _cut(g, p1) - _cut(g, _moved(p1, {u: dest[u]}))
+        before = c0 - _cut(g, _moved(part, {u: dest[u]}))
+        if gain[u] + (after - before) >= 0:
+            moves.append(u)
+    gh = sum(orc.mix64((orc.mix64(u) + (gain[u] & U64)) & U64) for u in gain) & U64
+    mh = sum(orc.mix64((u << 32) | dest[u]) for u in moves) & U64
+    return len(gain), moves, gh, mh, _cut(g, _moved(part, {u: dest[u] for u in moves}))
+
+
+TINY = [
+    ("rgg80w", lambda: heipa.gen_rgg(80, 7.0, 3, 9), 4),
+    ("grid9x8", lambda: heipa.gen_grid(9, 8), 3),
+    ("rmat6w", lambda: heipa.gen_rmat(6, 0, 6.0, 0.57, 0.19, 0.19, 5, 9, True), 5),
+]
+
+
+@pytest.mark.parametrize("name,make,k", TINY)
+@pytest.mark.parametrize("ultra,level,ratio", [(False, 0, 0.25), (False, 2, 0.75), (True, 0, 0.85)])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_jet_lp_first_iteration_brute_force(name, make, k, ultra, level, ratio, seed):
+    g = make()
+    part = random_partition(g.n, k, seed)
+    lmax = int(g.g_weight)  # never overloaded: the first iteration is LP
+    n_cand, moves, gh, mh, cut1 = _first_lp_iteration(g, part, k, ratio)
+    cut0 = orc.edge_cut(g, part)
+    _, _, _, tr = orc.jet_refine(g, part, k, lmax, ultra, level, cut0, int(block_weights(g, part, k).max()),
+                                 trace_cap=64)
+    assert moves, "a random partition of these graphs always has improving moves"
+    t = tr[0]
+    assert (t.ratio_idx, t.iteration, t.mode) == (0, 1, 0)
+    assert (t.n_cand, t.n_moves, t.gain_hash, t.moves_hash, t.cut) == (n_cand, len(moves), gh, mh, cut1)
+
+
+def test_jet_lp_ratio_changes_the_candidates():
+    """The three ratios give different candidate sets on one input, so a
+    wrong ratio for a level fails the test above."""
+    g = heipa.gen_rgg(80, 7.0, 3, 9)
+    part = random_partition(g.n, 4, 1)
+    counts = {r: _first_lp_iteration(g, part, 4, r)[0] for r in (0.25, 0.75, 0.85)}
+    assert len(set(counts.values())) == 3, counts
+
+
+def _conn(g, part, u, k, c0):
+    """Connectivity of u to every block, from cut evaluations: its own block's
+    is the cut increase of a move to an empty block, another block's is that
+    plus the cut drop of the move there."""
+    own = _cut(g, _moved(part, {u: k})) - c0
+    out = {int(part[u]): own}
+    for b in {int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]} - {int(part[u])}:
+        out[b] = own + c0 - _cut(g, _moved(part, {u: b}))
+    return out
+
+
+def _weak_iteration(g, part, k, lmax):
+    vw = (lambda u: 1) if g.vwgt is None else (lambda u: int(g.vwgt[u]))
+    W = int(g.g_weight)
+    bw = [int(x) for x in block_weights(g, part, k)]
+    opt = (W + k - 1) // k
+    max_b_w = max(int(lmax * 0.99), lmax - 100)
+    sections = 128
+    if (g.n + sections * k) // (sections * k) < 4096:
+        sections = (g.n + 4096 * k) // (4096 * k)
+    under = [b for b in range(k) if bw[b] < max_b_w]
+    c0 = _cut(g, part)
+    dest, key = {}, {}
+    for u in range(g.n):
+        src = int(part[u])
+        if not (bw[src] > lmax and vw(u) < 1.5 * (bw[src] - opt)):
+            continue
+        conn = _conn(g, part, u, k, c0)
+        own = conn[src]
+        room = [(c, b) for b, c in conn.items() if b != src and bw[b] < max_b_w and c > 0]
+        if room:
+            best_c = max(c for c, _ in room)
+            b = min(b for c, b in room if c == best_c)
+            gain = best_c - own
+        else:
+            b = under[u % len(under)]
+            gain = -own
+        dest[u] = b
+        key[u] = (src, refsim.gain_bucket_exact(gain, vw(u)), u % sections, u)
+    moves = []
+    for src in range(k):
+        before = 0
+        for u in sorted((u for u in dest if key[u][0] == src), key=lambda u: key[u]):
+            if before < bw[src] - lmax:
+                moves.append(u)
+            before += vw(u)
+    moves.sort()
+    return len(dest), moves, dest
+
+
+def _vertex_weighted(g, seed, hi):
+    vw = np.random.default_rng(seed).integers(1, hi + 1, g.n).astype(np.int32)
+    return heipa.HostGraph(g.xadj, g.adjncy, g.adjwgt, vw)
+
+
+WEAK = [
+    ("rgg120w", lambda: heipa.gen_rgg(120, 7.0, 3, 9), 4, 0.8),
+    ("grid10x9", lambda: heipa.gen_grid(10, 9), 3, 1.0),
+    ("rmat7w", lambda: heipa.gen_rmat(7, 0, 6.0, 0.57, 0.19, 0.19, 5, 9, True), 6, 0.6),
+    ("rgg150vw", lambda: _vertex_weighted(heipa.gen_rgg(150, 7.0, 4, 9), 4, 80), 4, 0.8),
+]
+
+
+@pytest.mark.parametrize("name,make,k,skew", WEAK)
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_jet_weak_first_iteration_brute_force(name, make, k, skew, seed):
+    g = make()
+    part = random_partition(g.n, k, seed, skew)
+    _check_weak(g, part, k)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_jet_weak_slight_overload_weighted(seed):
+    """Vertex weights 1..80; block 0 takes, in a random vertex order, every
+    vertex that still fits under lmax, then the lightest other vertex that
+    takes it over lmax; every other vertex goes to the lightest other block.
+    The heaviest vertices of block 0 are then no candidates (weight at least
+    1.5 (block weight - opt)), and which candidate moves first depends on the
+    gain per unit of weight."""
+    g = _vertex_weighted(heipa.gen_rgg(200, 7.0, 5, 9), 6, 80)
+    k = 8
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    w = [int(x) for x in g.vwgt]
+    order = [int(u) for u in np.random.default_rng(seed).permutation(g.n)]
+    part = np.full(g.n, k, np.uint32)
+    bw = np.zeros(k, np.int64)
+    for u in order:
+        if bw[0] + w[u] <= lmax:
+            part[u] = 0
+            bw[0] += w[u]
+    last = min((u for u in order if part[u] == k and bw[0] + w[u] > lmax), key=lambda u: (w[u], u))
+    part[last] = 0
+    bw[0] += w[last]
+    for u in order:
+        if part[u] == k:
+            b = 1 + int(np.argmin(bw[1:]))
+            part[u] = b
+            bw[b] += w[u]
+    opt = (int(g.g_weight) + k - 1) // k
+    assert (g.vwgt[part == 0] >= 1.5 * (bw[0] - opt)).any()
+    _check_weak(g, part, k)
+
+
+def test_jet_weak_full_neighbour_block_is_skipped():
+    """Row-major grid ids split 40 / 30 / 20 over k = 3: lmax = 31, and block 1
+    weighs 30 = floor(0.99 lmax), so it takes no vertex although it is below
+    lmax; block 0's vertices all go to block 2, the only one with room."""
+    g = heipa.gen_grid(10, 9)
+    part = np.array([0] * 40 + [1] * 30 + [2] * 20, np.uint32)
+    _check_weak(g, part, 3)
+
+
+def _check_weak(g, part, k):
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    bw = block_weights(g, part, k)
+    assert bw.max() > lmax  # the first iteration is a weak rebalance
+    _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), int(bw.max()), trace_cap=64)
+    assert (tr[0].ratio_idx, tr[0].iteration, tr[0].mode) == (0, 1, 1)
+    _check_iteration(g, tr[0], part, *_weak_iteration(g, part, k, lmax))
+
+
+def _check_iteration(g, t, part, n_cand, moves, dest):
+    """Compares one traced iteration with a brute-force move set; returns the
+    partition after the moves."""
+    assert moves
+    after = _moved(part, {u: dest[u] for u in moves})
+    mh = sum(orc.mix64((u << 32) | dest[u]) for u in moves) & U64
+    assert (t.n_cand, t.n_moves, t.moves_hash, t.cut) == (n_cand, len(moves), mh, _cut(g, after))
+    return after
+
+
+def _strong_iteration(g, part, k, lmax):
+    """rebalance_strong (jet:356-590).  Candidates: vertices of an overloaded
+    block no heavier than twice the largest room and lighter than twice the
+    block's excess over opt; score = mean connectivity to the blocks with room
+    minus the own connectivity, bucketed per min(weight, excess over lmax).
+    Each overloaded block evicts, in (bucket, id mod sections, id) order,
+    while the weight before a candidate is below its excess over lmax.  The
+    evicted vertices of all blocks, block by block in that order, are laid on
+    one line of weight positions, which the rooms of the blocks below
+    max_b_w cut into consecutive chunks; a vertex goes to the chunk that
+    holds its position if at least half its weight fits there, otherwise
+    (in id order) it is laid again at the end of the line, and it stays when
+    its position is past the last chunk."""
+    vw = (lambda u: 1) if g.vwgt is None else (lambda u: int(g.vwgt[u]))
+    W = int(g.g_weight)
+    bw = [int(x) for x in block_weights(g, part, k)]
+    opt = (W + k - 1) // k
+    max_b_w = max(opt + 1, int(lmax * 0.99))
+    sections = 128
+    if (g.n + sections * k) // (sections * k) < 4096:
+        sections = (g.n + 4096 * k) // (4096 * k)
+    rooms = [max_b_w - bw[b] for b in range(k) if bw[b] < max_b_w]
+    assert rooms
+    max_room = max(rooms)
+    c0 = _cut(g, part)
+    key = {}
+    for u in range(g.n):
+        src = int(part[u])
+        if not (bw[src] > lmax and vw(u) <= 2 * max_room and vw(u) < 2 * (bw[src] - opt)):
+            continue
+        conn = _conn(g, part, u, k, c0)
+        others = [c for b, c in conn.items() if b != src and bw[b] < max_b_w]
+        gain = sum(others) // max(len(others), 1) - conn[src]
+        key[u] = (src, refsim.gain_bucket_exact(gain, min(vw(u), bw[src] - lmax)), u % sections, u)
+    pos, line = {}, 0
+    for src in range(k):
+        before = 0
+        for u in sorted((u for u in key if key[u][0] == src), key=lambda u: key[u]):
+            if before < bw[src] - lmax:
+                pos[u] = line + before
+            before += vw(u)
+        line += sum(vw(u) for u in pos if key[u][0] == src)
+    ends, acc = [], 0
+    for b in range(k):
+        acc += max(0, max_b_w - bw[b])
+        ends.append(acc)
+    dest = {}
+    for u in sorted(pos):
+        x = pos[u]
+        while True:
+            chunk = next((b for b in range(k) if x < ends[b]), None)
+            if chunk is None:
+                dest[u] = int(part[u])
+                break
+            if vw(u) // 2 <= ends[chunk] - x:
+                dest[u] = chunk
+                break
+            x, line = line, line + vw(u)
+    return len(key), sorted(pos), dest
+
+
+STRONG = [
+    ("rgg120w", lambda: heipa.gen_rgg(120, 7.0, 3, 9), 8, 1.0, 1),
+    ("grid10x9", lambda: heipa.gen_grid(10, 9), 6, 1.0, 2),
+    ("rmat7w", lambda: heipa.gen_rmat(7, 0, 6.0, 0.57, 0.19, 0.19, 5, 9, True), 4, 1.0, 1),
+    ("rgg150vw", lambda: _vertex_weighted(heipa.gen_rgg(150, 7.0, 4, 9), 4, 80), 6, 3.0, 2),
+    ("rgg150vw", lambda: _vertex_weighted(heipa.gen_rgg(150, 7.0, 4, 9), 4, 80), 8, 3.0, 1),
+    ("rgg150vw", lambda: _vertex_weighted(heipa.gen_rgg(150, 7.0, 4, 9), 4, 80), 4, 1.0, 3),
+    ("rgg200vw20", lambda: _vertex_weighted(heipa.gen_rgg(200, 7.0, 5, 9), 6, 20), 3, 1.0, 3),
+]
+
+
+@pytest.mark.parametrize("name,make,k,skew,seed", STRONG)
+def test_jet_weak_weak_strong_chain_brute_force(name, make, k, skew, seed):
+    """Two weak iterations, then a strong one (jet:976-993), each recomputed
+    by brute force from the partition the previous one left."""
+    g = make()
+    part = random_partition(g.n, k, seed, skew)
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    bw = block_weights(g, part, k)
+    _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), int(bw.max()), trace_cap=64)
+    assert [t.mode for t in tr[:3]] == [1, 1, 2]
+    for t in tr[:2]:
+        part = _check_iteration(g, t, part, *_weak_iteration(g, part, k, lmax))
+    _check_iteration(g, tr[2], part, *_strong_iteration(g, part, k, lmax))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_jet_strong_takes_vertices_too_heavy_for_weak(seed):
+    """Block 0 holds 12 vertices of weight 60..75 and weighs opt + 39, so
+    every one of them is at least 1.5 (block weight - opt) (no weak
+    candidate: the weak iteration finds no move and the strong one runs
+    first) and below 2 (block weight - opt) (all strong candidates); their
+    weights exceed the block's excess over lmax, which then weights their
+    gain buckets.  Block 1 is light enough to take any of them."""
+    g0 = heipa.gen_rgg(200, 7.0, 6, 9)
+    rng = np.random.default_rng(seed)
+    heavy = rng.choice(g0.n, 12, replace=False)
+    vw = np.zeros(g0.n, np.int64)
+    vw[heavy] = rng.integers(60, 76, 12)
+    b0 = int(vw.sum())
+    opt = b0 - 39
+    rest = [u for u in range(g0.n) if vw[u] == 0]
+    vw[rest] = rng.integers(1, 21, len(rest))
+    vw[rest[-1]] += 4 * opt - int(vw.sum())  # total weight 4 opt
+    assert vw[rest[-1]] >= 1
+    g = heipa.HostGraph(g0.xadj, g0.adjncy, g0.adjwgt, vw.astype(np.int32))
+    k = 4
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    part = np.zeros(g.n, np.uint32)
+    bw = [b0, 0, 0, 0]
+    for u in rest:  # blocks 2 and 3 up to opt + 15, block 1 the rest
+        b = next((b for b in (2, 3) if bw[b] + vw[u] <= opt + 15), 1)
+        part[u] = b
+        bw[b] += int(vw[u])
+    assert (int(g.g_weight) + k - 1) // k == opt and bw[0] > lmax
+    assert all(1.5 * (bw[0] - opt) <= vw[u] < 2 * (bw[0] - opt) for u in heavy)
+    assert all(vw[u] > bw[0] - lmax for u in heavy) and max(bw[1:]) <= lmax
+    _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), bw[0], trace_cap=64)
+    assert tr[0].mode == 2
+    _check_iteration(g, tr[0], part, *_strong_iteration(g, part, k, lmax))
