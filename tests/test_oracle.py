@@ -247,6 +247,47 @@ def test_solver_invariants(kind, k, ultra):
     assert np.array_equal(p, p2) and cut == cut2
 
 
+@pytest.mark.parametrize("kind,k,ultra,imb", [("rgg", 4, False, 0.03), ("rmat", 6, True, 0.05),
+                                               ("grid", 3, False, 0.01)])
+def test_solver_level_loop_restated(kind, k, ultra, imb):
+    """internal_solve (solver.h:340-400) restated over the oracle's pinned
+    pieces: coarsen (two-hop matching with seed 12345, two_hop_matching.h:590)
+    while n > 8k, the host partitioner on the coarsest graph (no refinement
+    there), then per level from the coarsest-but-one down to 0: project the
+    partition through the level's mapping and run Jet with that level's index
+    and lmax = ceil((1 + imbalance) W / k) (solver.h:410), carrying the cut and
+    the maximum block weight from level to level."""
+    g = {"rgg": lambda: heipa.gen_rgg(4000, 9.0, 4, 10),
+         "rmat": lambda: heipa.gen_rmat(11, 0, 6.0, 0.57, 0.19, 0.19, 3, 20, True),
+         "grid": lambda: heipa.gen_grid(40, 45)}[kind]()
+    seed = 5
+    lmax = math.ceil((1.0 + imb) * (int(g.g_weight) / k))
+    graphs, maps = [g], []
+    while graphs[-1].n > 8 * k:
+        mp, nc, _ = orc.match(graphs[-1], 12345)
+        cx, ca, cw, cv, rc = orc.contract(graphs[-1], mp, nc)
+        assert rc == 0
+        maps.append(mp)
+        graphs.append(heipa.HostGraph(cx, ca, cw, cv))
+    assert len(maps) >= 2
+    part = heipa.initial_partition(graphs[-1], k, imb, seed)
+    cut, maxw = orc.edge_cut(graphs[-1], part), int(block_weights(graphs[-1], part, k).max())
+    levels = {len(maps): (cut, maxw)}
+    for lv in range(len(maps) - 1, -1, -1):
+        part = part[maps[lv]]
+        part, cut, maxw, _ = orc.jet_refine(graphs[lv], part, k, lmax, ultra, lv, cut, maxw)
+        levels[lv] = (cut, maxw)
+    p, c, mw, lt = orc.partition(g, k, imb, seed, ultra)
+    assert np.array_equal(p, part) and (c, mw) == (cut, maxw)
+    assert len(lt) == len(graphs)
+    for lv, t in enumerate(lt):
+        assert (t.n, t.m) == (graphs[lv].n, len(graphs[lv].adjncy))
+        assert (t.cut, t.maxw) == levels[lv]
+        if lv < len(maps):
+            assert t.nc == graphs[lv + 1].n
+            assert t.mapping_hash == sum(orc.mix64((u << 32) | int(maps[lv][u])) for u in range(graphs[lv].n)) & ((1 << 64) - 1)
+
+
 def test_grid_quality_near_optimal():
     # 64x64 grid, k=8: a 2x4 tiling cuts 4*64 = 256 edges; the result must be close
     g = heipa.gen_grid(64, 64)

@@ -141,8 +141,10 @@ def test_map_refine_uniform_distance_is_twice_cut():
     assert J == 2 * orc.edge_cut(g, p)
 
 
-def _gms_python(g, hier, eps, seed):
-    """global_multisection_host (global_multisection.h:139-259), restated again."""
+def _gms_python(g, hier, eps, seed, split=None):
+    """global_multisection_host (global_multisection.h:139-259), restated again;
+    `split(subgraph, k, imbalance)` replaces the host partitioner (the
+    hierarchical multisection passes the multilevel Solver)."""
     l = len(hier)
     gk = int(np.prod(hier))
     index_vec = [1]
@@ -160,7 +162,7 @@ def _gms_python(g, hier, eps, seed):
         lw = int(sg.vwgt.astype(np.int64).sum()) if sg.vwgt is not None else sg.n
         depth = l - len(ident)
         imb = ((1.0 + eps) * (k_rem[l - 1 - len(ident)] * W) / (gk * lw)) ** (1.0 / depth) - 1.0
-        tmp = heipa.initial_partition(sg, k, imb, seed)
+        tmp = heipa.initial_partition(sg, k, imb, seed) if split is None else split(sg, k, imb)
         if len(ident) == l - 1:
             off = sum(ident[i] * index_vec[len(index_vec) - 1 - i] for i in range(l - 1))
             out[n_to_o] = off + tmp
@@ -192,6 +194,26 @@ def test_global_multisection_python_restatement(hier):
     g = heipa.gen_rgg(1500, 8.0, 11, 10)
     a = orc.global_multisection(g, hier, 0.03, 7)
     b = _gms_python(g, hier, 0.03, 7)
+    assert np.array_equal(a, b)
+    assert a.max() < np.prod(hier)
+
+
+@pytest.mark.parametrize("hier,ultra", [([2, 3, 2], False), ([3, 1, 2], True), ([4, 2], False)])
+def test_hm_partition_python_restatement(hier, ultra):
+    """recursive_multisection_device (hierarchical_multisection.h:56-235): the
+    same recursion, child graphs and leaf offsets as the global multisection,
+    with every split a full multilevel solve (gpu_heipa_partition :38-54, all
+    zeros for k = 1) under the adaptive imbalance.  The Solver itself is
+    pinned by its own restatement in test_oracle.py."""
+    g = heipa.gen_rmat(11, 0, 8.0, 0.57, 0.19, 0.19, 4, 10, True)
+
+    def solve(sg, k, imb):
+        if k == 1:
+            return np.zeros(sg.n, np.uint32)
+        return orc.partition(sg, k, imb, 7, ultra)[0]
+
+    a = orc.hm_partition(g, hier, 0.03, 7, ultra)
+    b = _gms_python(g, hier, 0.03, 7, solve)
     assert np.array_equal(a, b)
     assert a.max() < np.prod(hier)
 
