@@ -239,6 +239,40 @@ def test_im_partition_invariants(kind, hier, dist):
     assert np.array_equal(p, p2) and J == J2
 
 
+@pytest.mark.parametrize("kind,hier,dist,imb", [("rgg", [2, 3, 2], [1, 10, 100], 0.03),
+                                                ("rmat", [3, 4], [1, 9], 0.05)])
+def test_im_level_loop_restated(kind, hier, dist, imb):
+    """ProMapSolver::internal_solve (promap_solver.h:216-246, :299, :336-347)
+    restated over the pinned pieces: coarsen while n > 8k, the global
+    multisection of the coarsest graph (no refinement there), then per level
+    projection and the mapping Jet under lmax = ceil((1 + imbalance) W / k),
+    J and the maximum block weight carried from level to level."""
+    g = {"rgg": lambda: heipa.gen_rgg(3000, 9.0, 2, 10),
+         "rmat": lambda: heipa.gen_rmat(10, 0, 8.0, 0.57, 0.19, 0.19, 6, 20, True)}[kind]()
+    seed = 4
+    k = int(np.prod(hier))
+    D = orc.distance_matrix(hier, dist)
+    lmax = math.ceil((1.0 + imb) * (int(g.g_weight) / k))
+    graphs, maps = [g], []
+    while graphs[-1].n > 8 * k:
+        mp, nc, _ = orc.match(graphs[-1], 12345)
+        cx, ca, cw, cv, rc = orc.contract(graphs[-1], mp, nc)
+        assert rc == 0
+        maps.append(mp)
+        graphs.append(heipa.HostGraph(cx, ca, cw, cv))
+    assert len(maps) >= 2
+    part = orc.global_multisection(graphs[-1], hier, imb, seed)
+    J, maxw = orc.comm_cost(graphs[-1], part, D), int(block_weights(graphs[-1], part, k).max())
+    levels = {len(maps): (J, maxw)}
+    for lv in range(len(maps) - 1, -1, -1):
+        part = part[maps[lv]]
+        part, J, maxw, _ = orc.map_refine(graphs[lv], part, D, lmax, J, maxw)
+        levels[lv] = (J, maxw)
+    p, J1, mw1, lt = orc.im_partition(g, hier, dist, imb, seed)
+    assert np.array_equal(p, part) and (J1, mw1) == (J, maxw)
+    assert [(t.n, t.cut, t.maxw) for t in lt] == [(graphs[lv].n, *levels[lv]) for lv in range(len(graphs))]
+
+
 @pytest.mark.parametrize("hier", [[4, 8, 6], [2, 3, 2], [3, 1, 2]])
 def test_product_global_multisection_matches_oracle(hier):
     # the library's host routine (no GPU involved) against the oracle's restatement

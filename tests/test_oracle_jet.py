@@ -26,6 +26,10 @@ the selection is stated as a sort, not as the bucket prefix sums the oracle
 and the reference use: in every overloaded block the candidates are ordered by
 (gain bucket, id mod sections, id) and a candidate moves while the weight of
 the candidates before it is below the block's excess over lmax.
+
+Whole jet_refine calls (jet:926-1033) are then run over these brute-force
+iterations with the reference's decision cache and locks, and every traced
+iteration and the result must agree with the oracle.
 """
 import math
 
@@ -152,9 +156,11 @@ def _weak_iteration(g, part, k, lmax):
             best_c = max(c for c, _ in room)
             b = min(b for c, b in room if c == best_c)
             gain = best_c - own
-        else:
+        elif under:
             b = under[u % len(under)]
             gain = -own
+        else:
+            continue  # the reference divides by zero here (jet:689); the vertex stays
         dest[u] = b
         key[u] = (src, refsim.gain_bucket_exact(gain, vw(u)), u % sections, u)
     moves = []
@@ -271,7 +277,8 @@ def _strong_iteration(g, part, k, lmax):
     if (g.n + sections * k) // (sections * k) < 4096:
         sections = (g.n + 4096 * k) // (4096 * k)
     rooms = [max_b_w - bw[b] for b in range(k) if bw[b] < max_b_w]
-    assert rooms
+    if not rooms:
+        return 0, [], {}
     max_room = max(rooms)
     c0 = _cut(g, part)
     key = {}
@@ -370,3 +377,125 @@ def test_jet_strong_takes_vertices_too_heavy_for_weak(seed):
     _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), bw[0], trace_cap=64)
     assert tr[0].mode == 2
     _check_iteration(g, tr[0], part, *_strong_iteration(g, part, k, lmax))
+
+
+def _lp_iteration(g, part, k, ratio, cache, locked):
+    """One LP iteration with the reference's decision cache (jet:193-238: a
+    vertex's destination and gain, or its lack of one, are recomputed only
+    after it or a neighbour moved, otherwise reused even under a later
+    ratio) and locks (jet:247-258: a vertex moved by the previous LP
+    iteration is no candidate in this one).  Decisions and the afterburner
+    come from cut evaluations as in the first-iteration test."""
+    c0 = _cut(g, part)
+    for u in range(g.n):
+        if u in cache:
+            continue
+        own = _cut(g, _moved(part, {u: k})) - c0
+        best, best_b = None, None
+        for b in sorted({int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]} - {int(part[u])}):
+            drop = c0 - _cut(g, _moved(part, {u: b}))
+            if best is None or drop > best:
+                best, best_b = drop, b
+        keep = best_b is not None and (best >= 0 or -best < math.floor(ratio * own))
+        cache[u] = (best, best_b) if keep else (0, None)
+    gain = {u: cache[u][0] for u in range(g.n) if cache[u][1] is not None and u not in locked}
+    dest = {u: cache[u][1] for u in gain}
+    moves = []
+    for u in sorted(gain):
+        first = {int(v): dest[int(v)] for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]
+                 if int(v) in gain and (gain[int(v)] > gain[u] or (gain[int(v)] == gain[u] and v < u))}
+        p1 = _moved(part, first)
+        after = _cut(g, p1) - _cut(g, _moved(p1, {u: dest[u]}))
+        before = c0 - _cut(g, _moved(part, {u: dest[u]}))
+        if gain[u] + (after - before) >= 0:
+            moves.append(u)
+    gh = sum(orc.mix64((orc.mix64(u) + (gain[u] & U64)) & U64) for u in gain) & U64
+    return len(gain), moves, dest, gh
+
+
+ULTRA = [0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45, 0.40, 0.35, 0.30, 0.25, 0.20, 0.15, 0.10, 0.05]
+
+
+def _jet_refine(g, part, k, lmax, ultra, level):
+    """jet_refine (jet:926-1033) over the brute-force iterations: per filter
+    ratio at most 12 iterations; LP while balanced (no moves ends the
+    ratio), else up to 2 weak iterations (no moves: go to strong) then strong
+    (no moves ends the ratio); after each move set, a lower maximum block
+    weight while the best is overloaded, or a lower cut at a balanced or no
+    heavier state, becomes the best, and the first case or a cut below
+    0.999 of the best restarts the iteration count."""
+    w = np.ones(g.n, np.int64) if g.vwgt is None else g.vwgt.astype(np.int64)
+    cut = _cut(g, part)
+    maxw = int(np.bincount(part, weights=w, minlength=k).max())
+    best = (part.copy(), cut, maxw)
+    cache, locked, trace = {}, set(), []
+    for ri, ratio in enumerate(ULTRA if ultra else [0.25 if level == 0 else 0.75]):
+        bal = it = 0
+        while it < 12:
+            it += 1
+            gh = 0
+            if maxw <= lmax:
+                mode = 0
+                n_cand, moves, dest, gh = _lp_iteration(g, part, k, ratio, cache, locked)
+                locked = set(moves)
+                bal = 0
+                if not moves:
+                    break
+            else:
+                if bal < 2:
+                    mode = 1
+                    n_cand, moves, dest = _weak_iteration(g, part, k, lmax)
+                    if not moves:
+                        bal = 2
+                        continue
+                else:
+                    mode = 2
+                    n_cand, moves, dest = _strong_iteration(g, part, k, lmax)
+                    if not moves:
+                        break
+                bal += 1
+            mh = sum(orc.mix64((u << 32) | dest[u]) for u in moves) & U64
+            for u in moves:
+                cache.pop(u, None)
+                for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]:
+                    cache.pop(int(v), None)
+            part = _moved(part, {u: dest[u] for u in moves})
+            cut = _cut(g, part)
+            maxw = int(np.bincount(part, weights=w, minlength=k).max())
+            trace.append((ri, it, mode, n_cand, len(moves), cut, maxw, mh, gh))
+            if best[2] > lmax and maxw < best[2]:
+                best = (part.copy(), cut, maxw)
+                it = 0
+            elif cut < best[1] and (maxw <= lmax or maxw <= best[2]):
+                if cut < 0.999 * best[1]:
+                    it = 0
+                best = (part.copy(), cut, maxw)
+    return best, trace
+
+
+LOOP = [
+    ("rgg60w", lambda: heipa.gen_rgg(60, 6.0, 3, 9), 4, False, 0, 0.8, 1),
+    ("rgg60w-coarse", lambda: heipa.gen_rgg(60, 6.0, 3, 9), 3, False, 1, 0.0, 2),
+    ("grid8x8-ultra", lambda: heipa.gen_grid(8, 8), 4, True, 0, 1.0, 3),
+    ("rgg70vw-ultra", lambda: _vertex_weighted(heipa.gen_rgg(70, 6.0, 4, 9), 4, 40), 3, True, 0, 3.0, 2),
+    # a lower cut at an overloaded state as heavy as the best one becomes the
+    # best (jet:1010, `<=`)
+    ("grid8x8-k6", lambda: heipa.gen_grid(8, 8), 6, False, 0, 1.0, 2),
+    ("rgg50vw-k6", lambda: _vertex_weighted(heipa.gen_rgg(50, 5.0, 7, 9), 5, 60), 6, False, 0, 0.0, 3),
+]
+
+
+@pytest.mark.parametrize("name,make,k,ultra,level,skew,seed", LOOP)
+def test_jet_refine_loop_brute_force(name, make, k, ultra, level, skew, seed):
+    """Every iteration of a whole jet_refine call, and its result."""
+    g = make()
+    part = random_partition(g.n, k, seed, skew)
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    (bp, bc, bm), trace = _jet_refine(g, part, k, lmax, ultra, level)
+    cut0, maxw0 = orc.edge_cut(g, part), int(block_weights(g, part, k).max())
+    p, c, mw, tr = orc.jet_refine(g, part, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
+    got = [(t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
+           for t in tr]
+    assert len(trace) > 3
+    assert got == trace
+    assert np.array_equal(p, bp) and (c, mw) == (bc, bm)
