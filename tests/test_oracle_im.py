@@ -21,6 +21,7 @@ import pytest
 
 import heipa
 import heipa_oracle as orc
+from brute import heavy_block_case, strong_iteration, vertex_weighted, weak_iteration
 from graphs import block_weights, random_partition
 
 GAIN_MIN = -(1 << 63)
@@ -93,6 +94,128 @@ def test_map_lp_first_iteration_brute_force(name, make, hier, dist, seed):
     t = tr[0]
     assert (t.iteration, t.mode) == (1, 0)
     assert (t.n_cand, t.n_moves, t.gain_hash, t.moves_hash, t.cut) == (n_cand, len(moves), gh, mh, J1)
+
+
+def _map_lp_iteration(g, part, D, cache, locked):
+    """One ProMap LP iteration (promap:206-330) with the decision cache and
+    locks of the reference (a decision is recomputed only after the vertex or
+    a neighbour moved; a vertex moved by the previous LP iteration is no
+    candidate), decisions and afterburner from whole-graph J evaluations."""
+    J0 = _J(g, part, D)
+    for u in range(g.n):
+        if u in cache:
+            continue
+        best, best_b = -1, None
+        for b in sorted({int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]} - {int(part[u])}):
+            d2 = J0 - _J(g, _moved(part, {u: b}), D)
+            if d2 // 2 > best:
+                best, best_b = d2 // 2, b
+        cache[u] = (best, best_b) if best_b is not None else (0, None)
+    gain = {u: cache[u][0] for u in range(g.n) if cache[u][1] is not None and u not in locked}
+    dest = {u: cache[u][1] for u in gain}
+    moves = []
+    for u in sorted(gain):
+        first = {int(v): dest[int(v)] for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]
+                 if int(v) in gain and (gain[int(v)] > gain[u] or (gain[int(v)] == gain[u] and v < u))}
+        p1 = _moved(part, first)
+        after = (_J(g, p1, D) - _J(g, _moved(p1, {u: dest[u]}), D)) // 2
+        before = (J0 - _J(g, _moved(part, {u: dest[u]}), D)) // 2
+        if gain[u] + (after - before) >= 0:
+            moves.append(u)
+    gh = sum(orc.mix64((orc.mix64(u) + (gain[u] & U64)) & U64) for u in gain) & U64
+    return len(gain), moves, dest, gh
+
+
+def _map_refine(g, part, D, lmax):
+    """ProMap_jet_refine (promap:1219-1311) over brute-force iterations: at
+    most 12 iterations; LP while balanced, else the balance counter goes up
+    first and its first value runs a weak iteration (the rebalancing is the
+    plain one, promap:333-718 differing from jet:356-751 only in the order of
+    the candidate list), later ones strong; an iteration without moves is
+    skipped; a lower maximum block weight while the best is overloaded, or a
+    lower J at a balanced or strictly lighter state, becomes the best."""
+    k = D.shape[0]
+    w = np.ones(g.n, np.int64) if g.vwgt is None else g.vwgt.astype(np.int64)
+    J = _J(g, part, D)
+    maxw = int(np.bincount(part, weights=w, minlength=k).max())
+    best = (part.copy(), J, maxw)
+    cache, locked, trace = {}, set(), []
+    bal = it = 0
+    while it < 12:
+        it += 1
+        gh = 0
+        if maxw <= lmax:
+            mode = 0
+            n_cand, moves, dest, gh = _map_lp_iteration(g, part, D, cache, locked)
+            locked = set(moves)
+            bal = 0
+        else:
+            bal += 1
+            mode = 1 if bal < 2 else 2
+            n_cand, moves, dest = (weak_iteration if mode == 1 else strong_iteration)(g, part, k, lmax)
+        if not moves:
+            continue
+        mh = sum(orc.mix64((u << 32) | dest[u]) for u in moves) & U64
+        for u in moves:
+            cache.pop(u, None)
+            for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]:
+                cache.pop(int(v), None)
+        part = _moved(part, {u: dest[u] for u in moves})
+        J = _J(g, part, D)
+        maxw = int(np.bincount(part, weights=w, minlength=k).max())
+        trace.append((it, mode, n_cand, len(moves), J, maxw, mh, gh))
+        if best[2] > lmax and maxw < best[2]:
+            best = (part.copy(), J, maxw)
+            it = 0
+        elif J < best[1] and (maxw <= lmax or maxw < best[2]):
+            if J < 0.999 * best[1]:
+                it = 0
+            best = (part.copy(), J, maxw)
+    return best, trace
+
+
+MAP_LOOP = [
+    ("rgg60w", lambda: heipa.gen_rgg(60, 6.0, 3, 9), [2, 2], [1, 10], 0.8, 1),
+    ("grid8x8", lambda: heipa.gen_grid(8, 8), [3, 2], [1, 7], 1.0, 2),
+    ("rgg70vw", lambda: vertex_weighted(heipa.gen_rgg(70, 6.0, 4, 9), 4, 40), [2, 3], [2, 5], 3.0, 3),
+    ("rmat6w", lambda: heipa.gen_rmat(6, 0, 6.0, 0.57, 0.19, 0.19, 5, 9, True), [2, 2, 2], [1, 10, 100], 0.0, 1),
+    # a lower J at an overloaded state as heavy as the best one does not
+    # become the best (promap:1290, strict `<`)
+    ("grid8x8-k6", lambda: heipa.gen_grid(8, 8), [3, 2], [1, 5], 1.0, 1),
+    ("rgg50vw-k6", lambda: vertex_weighted(heipa.gen_rgg(50, 5.0, 7, 9), 5, 60), [2, 3], [2, 9], 0.0, 3),
+]
+
+
+@pytest.mark.parametrize("name,make,hier,dist,skew,seed", MAP_LOOP)
+def test_map_refine_loop_brute_force(name, make, hier, dist, skew, seed):
+    """Every iteration of a whole ProMap_jet_refine call, and its result."""
+    g = make()
+    D = orc.distance_matrix(hier, dist)
+    k = D.shape[0]
+    part = random_partition(g.n, k, seed, skew)
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    (bp, bJ, bm), trace = _map_refine(g, part, D, lmax)
+    J0, maxw0 = orc.comm_cost(g, part, D), int(block_weights(g, part, k).max())
+    p, J, mw, tr = orc.map_refine(g, part, D, lmax, J0, maxw0, trace_cap=4096)
+    got = [(t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash) for t in tr]
+    assert len(trace) > 3
+    assert got == trace
+    assert np.array_equal(p, bp) and (J, mw) == (bJ, bm)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_map_refine_loop_skips_an_iteration_without_moves(seed):
+    """The weak iteration finds no move in a block only the strong iteration
+    may empty (heavy_block_case); the loop skips it and goes on
+    (promap:1276)."""
+    g, part, k, lmax, bw = heavy_block_case(seed)
+    D = orc.distance_matrix([2, 2], [1, 10])
+    (bp, bJ, bm), trace = _map_refine(g, part, D, lmax)
+    p, J, mw, tr = orc.map_refine(g, part, D, lmax, orc.comm_cost(g, part, D), bw[0], trace_cap=4096)
+    got = [(t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash) for t in tr]
+    assert trace[0][:2] == (2, 2)  # iteration 1 (weak) had no move
+    assert got == trace
+    assert np.array_equal(p, bp) and (J, mw) == (bJ, bm)
 
 
 GRAPHS = [
