@@ -177,3 +177,32 @@ def heavy_block_case(seed):
     assert all(1.5 * (bw[0] - opt) <= vw[u] < 2 * (bw[0] - opt) for u in heavy)
     assert all(vw[u] > bw[0] - lmax for u in heavy) and max(bw[1:]) <= lmax
     return g, part, k, lmax, bw
+
+
+def slight_overload_case(seed):
+    """Vertex weights 1..80; block 0 takes, in a random vertex order, every
+    vertex that still fits under lmax, then the lightest other vertex that
+    takes it over lmax; every other vertex goes to the lightest other block.
+    The heaviest vertices of block 0 are then no candidates (weight at least
+    1.5 (block weight - opt)), and which candidate moves first depends on the
+    gain per unit of weight."""
+    g = vertex_weighted(heipa.gen_rgg(200, 7.0, 5, 9), 6, 80)
+    k = 8
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    w = [int(x) for x in g.vwgt]
+    order = [int(u) for u in np.random.default_rng(seed).permutation(g.n)]
+    part = np.full(g.n, k, np.uint32)
+    bw = np.zeros(k, np.int64)
+    for u in order:
+        if bw[0] + w[u] <= lmax:
+            part[u] = 0
+            bw[0] += w[u]
+    last = min((u for u in order if part[u] == k and bw[0] + w[u] > lmax), key=lambda u: (w[u], u))
+    part[last] = 0
+    bw[0] += w[last]
+    for u in order:
+        if part[u] == k:
+            b = 1 + int(np.argmin(bw[1:]))
+            part[u] = b
+            bw[b] += w[u]
+    return g, part, k, lmax, bw

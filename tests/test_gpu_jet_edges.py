@@ -1,0 +1,119 @@
+"""GPU parity of Jet on the inputs the oracle's brute-force pins were built
+around (rebalancing corner cases and whole refine calls on small graphs), in
+every device form of Jet: one workgroup with its state in LDS or in device
+memory, and the multi-launch form replayed from HIP graphs or launched
+kernel by kernel.  Every traced iteration and the result must equal the
+oracle's, bit for bit."""
+import math
+
+import numpy as np
+import pytest
+
+import heipa
+import heipa_oracle as orc
+from brute import heavy_block_case, slight_overload_case, vertex_weighted
+from graphs import block_weights, random_partition
+from test_gpu_parity import _dev, _trace_tuple
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {
+    "block-lds": {heipa.OPT_BLOCK_JET: 1 << 20, heipa.OPT_BLOCK_LDS: 1},
+    "block-mem": {heipa.OPT_BLOCK_JET: 1 << 20, heipa.OPT_BLOCK_LDS: 0},
+    "multi-graph": {heipa.OPT_BLOCK_JET: 0, heipa.OPT_GRAPHS: 1},
+    "multi-eager": {heipa.OPT_BLOCK_JET: 0, heipa.OPT_GRAPHS: 0},
+}
+DEFAULTS = {heipa.OPT_BLOCK_JET: 2048, heipa.OPT_BLOCK_LDS: 1, heipa.OPT_GRAPHS: 1}
+
+
+def _random_case(make, k, skew, seed):
+    g = make()
+    return g, random_partition(g.n, k, seed, skew), k, math.ceil(1.03 * int(g.g_weight) / k)
+
+
+def _ranges(g, sizes, lmax):
+    """Blocks of consecutive vertex ids with the given sizes."""
+    part = np.repeat(np.arange(len(sizes)), sizes).astype(np.uint32)
+    assert len(part) == g.n
+    return g, part, len(sizes), lmax
+
+
+def _cases():
+    out = {}
+    for s in (1, 2, 3):
+        out[f"heavy{s}"] = heavy_block_case(s)[:4]  # weak finds nothing, strong runs
+    for s in (1, 2):
+        out[f"slight{s}"] = slight_overload_case(s)[:4]  # too heavy for weak, buckets per weight
+    # a neighbour block at floor(0.99 lmax) takes nothing
+    out["fullnb"] = _ranges(heipa.gen_grid(10, 9), [40, 30, 20], 31)
+    # no block below max_b_w = 298 for the weak iteration (every vertex stays),
+    # but two below the strong iteration's max(opt + 1, 298) = 301
+    out["nounder"] = _ranges(heipa.gen_grid(30, 30), [303, 299, 298], 302)
+    out["rgg60w"] = _random_case(lambda: heipa.gen_rgg(60, 6.0, 3, 9), 4, 0.8, 1)
+    out["grid8k6"] = _random_case(lambda: heipa.gen_grid(8, 8), 6, 1.0, 2)
+    out["rgg70vw"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(70, 6.0, 4, 9), 4, 40), 3, 3.0, 2)
+    out["rgg50vwk6"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(50, 5.0, 7, 9), 5, 60), 6, 0.0, 3)
+    return out
+
+
+CASES = _cases()
+
+
+def _with_form(ctx, form, fn):
+    for opt, v in FORMS[form].items():
+        ctx.set_option(opt, v)
+    try:
+        fn()
+    finally:
+        for opt, v in DEFAULTS.items():
+            ctx.set_option(opt, v)
+
+
+def _same(name, gtr, otr):
+    assert len(gtr) == len(otr), (name, len(gtr), len(otr))
+    for i, (a, b) in enumerate(zip(gtr, otr)):
+        assert _trace_tuple(a) == _trace_tuple(b), (name, i, _trace_tuple(a), _trace_tuple(b))
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+@pytest.mark.parametrize("ultra,level", [(False, 0), (False, 1), (True, 0)])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_jet_edges(ctx, name, ultra, level, form):
+    g, part0, k, lmax = CASES[name]
+    cut0, maxw0 = orc.edge_cut(g, part0), int(block_weights(g, part0, k).max())
+    op, ocut, omaxw, otr = orc.jet_refine(g, part0, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
+    print(f"jet edges {name} ultra={ultra} level={level} form={form} n={g.n} k={k}", flush=True)
+
+    def run():
+        dg = _dev(g)
+        pd = heipa.from_numpy_u32(part0)
+        gcut, gmaxw, gtr = ctx.jet_refine(dg, pd, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
+        _same(name, gtr, otr)
+        assert (gcut, gmaxw) == (ocut, omaxw)
+        assert np.array_equal(heipa.to_numpy_u32(pd), op), name
+
+    _with_form(ctx, form, run)
+
+
+MAP_HIERS = {3: ([3], [4]), 4: ([2, 2], [1, 10]), 6: ([3, 2], [1, 5]), 8: ([2, 2, 2], [1, 10, 100])}
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_map_refine_edges(ctx, name, form):
+    g, part0, k, lmax = CASES[name]
+    D = orc.distance_matrix(*MAP_HIERS[k])
+    Dd = heipa.from_numpy_u32(D.reshape(-1).view(np.uint32))
+    J0, maxw0 = orc.comm_cost(g, part0, D), int(block_weights(g, part0, k).max())
+    op, oJ, omaxw, otr = orc.map_refine(g, part0, D, lmax, J0, maxw0, trace_cap=4096)
+    print(f"map edges {name} form={form} n={g.n} k={k}", flush=True)
+
+    def run():
+        dg = _dev(g)
+        pd = heipa.from_numpy_u32(part0)
+        gJ, gmaxw, gtr = ctx.map_refine(dg, pd, Dd, k, lmax, 0, J0, maxw0, trace_cap=4096)
+        _same(name, gtr, otr)
+        assert (gJ, gmaxw) == (oJ, omaxw)
+        assert np.array_equal(heipa.to_numpy_u32(pd), op), name
+
+    _with_form(ctx, form, run)

@@ -38,7 +38,7 @@ import pytest
 
 import heipa
 import heipa_oracle as orc
-from brute import U64, cut as _cut, heavy_block_case, moved as _moved, strong_iteration as _strong_iteration, \
+from brute import U64, cut as _cut, heavy_block_case, moved as _moved, slight_overload_case, strong_iteration as _strong_iteration, \
     vertex_weighted as _vertex_weighted, weak_iteration as _weak_iteration
 from graphs import block_weights, random_partition
 
@@ -124,31 +124,10 @@ def test_jet_weak_first_iteration_brute_force(name, make, k, skew, seed):
 
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 def test_jet_weak_slight_overload_weighted(seed):
-    """Vertex weights 1..80; block 0 takes, in a random vertex order, every
-    vertex that still fits under lmax, then the lightest other vertex that
-    takes it over lmax; every other vertex goes to the lightest other block.
-    The heaviest vertices of block 0 are then no candidates (weight at least
-    1.5 (block weight - opt)), and which candidate moves first depends on the
-    gain per unit of weight."""
-    g = _vertex_weighted(heipa.gen_rgg(200, 7.0, 5, 9), 6, 80)
-    k = 8
-    lmax = math.ceil(1.03 * int(g.g_weight) / k)
-    w = [int(x) for x in g.vwgt]
-    order = [int(u) for u in np.random.default_rng(seed).permutation(g.n)]
-    part = np.full(g.n, k, np.uint32)
-    bw = np.zeros(k, np.int64)
-    for u in order:
-        if bw[0] + w[u] <= lmax:
-            part[u] = 0
-            bw[0] += w[u]
-    last = min((u for u in order if part[u] == k and bw[0] + w[u] > lmax), key=lambda u: (w[u], u))
-    part[last] = 0
-    bw[0] += w[last]
-    for u in order:
-        if part[u] == k:
-            b = 1 + int(np.argmin(bw[1:]))
-            part[u] = b
-            bw[b] += w[u]
+    """Block 0 just over lmax with vertex weights 1..80 (slight_overload_case):
+    its heaviest vertices are no candidates, and which candidate moves first
+    depends on the gain per unit of weight."""
+    g, part, k, lmax, bw = slight_overload_case(seed)
     opt = (int(g.g_weight) + k - 1) // k
     assert (g.vwgt[part == 0] >= 1.5 * (bw[0] - opt)).any()
     _check_weak(g, part, k)
@@ -337,5 +316,23 @@ def test_jet_refine_loop_brute_force(name, make, k, ultra, level, skew, seed):
     got = [(t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
            for t in tr]
     assert len(trace) > 3
+    assert got == trace
+    assert np.array_equal(p, bp) and (c, mw) == (bc, bm)
+
+
+def test_jet_refine_loop_without_underloaded_block():
+    """Blocks of 303 / 299 / 298 consecutive ids of a 30x30 grid, lmax = 302:
+    no block is below the weak iteration's max_b_w = 298, so every vertex
+    stays (the reference divides by zero there, jet:689) and the weak
+    iteration finds no move; the strong iteration, whose max_b_w is
+    max(opt + 1, 298) = 301, then runs."""
+    g = heipa.gen_grid(30, 30)
+    part = np.repeat(np.arange(3), [303, 299, 298]).astype(np.uint32)
+    k, lmax = 3, 302
+    (bp, bc, bm), trace = _jet_refine(g, part, k, lmax, False, 0)
+    p, c, mw, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), 303, trace_cap=64)
+    got = [(t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
+           for t in tr]
+    assert trace[0][1:3] == (2, 2)
     assert got == trace
     assert np.array_equal(p, bp) and (c, mw) == (bc, bm)
