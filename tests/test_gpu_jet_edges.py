@@ -53,6 +53,10 @@ def _cases():
     out["grid8k6"] = _random_case(lambda: heipa.gen_grid(8, 8), 6, 1.0, 2)
     out["rgg70vw"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(70, 6.0, 4, 9), 4, 40), 3, 3.0, 2)
     out["rgg50vwk6"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(50, 5.0, 7, 9), 5, 60), 6, 0.0, 3)
+    # lmax within 0.1-1% of W / k: strong evicts by min(weight, excess)
+    for k, eps, skew, seed in ((4, 0.001, 0.5, 1), (2, 0.003, 2.0, 1), (3, 0.01, 2.0, 2)):
+        g = vertex_weighted(heipa.gen_grid(8, 8), 3, 20)
+        out[f"tight{k}"] = (g, random_partition(g.n, k, seed, skew), k, math.ceil((1 + eps) * int(g.g_weight) / k))
     return out
 
 
@@ -95,7 +99,7 @@ def test_jet_edges(ctx, name, ultra, level, form):
     _with_form(ctx, form, run)
 
 
-MAP_HIERS = {3: ([3], [4]), 4: ([2, 2], [1, 10]), 6: ([3, 2], [1, 5]), 8: ([2, 2, 2], [1, 10, 100])}
+MAP_HIERS = {2: ([2], [3]), 3: ([3], [4]), 4: ([2, 2], [1, 10]), 6: ([3, 2], [1, 5]), 8: ([2, 2, 2], [1, 10, 100])}
 
 
 @pytest.mark.parametrize("form", sorted(FORMS))

@@ -336,3 +336,22 @@ def test_jet_refine_loop_without_underloaded_block():
     assert trace[0][1:3] == (2, 2)
     assert got == trace
     assert np.array_equal(p, bp) and (c, mw) == (bc, bm)
+
+
+@pytest.mark.parametrize("k,eps,skew,seed", [(4, 0.001, 0.5, 1), (2, 0.003, 2.0, 1), (3, 0.01, 2.0, 2)])
+def test_jet_refine_loop_tight_lmax(k, eps, skew, seed):
+    """An 8x8 grid with vertex weights 1..20 and lmax = ceil((1 + eps) W / k)
+    with eps of 0.1-1%: strong iterations meet blocks whose excess over lmax
+    is below their candidates' weights, so the bucket weight
+    min(weight, excess) (jet:404) decides which candidate is evicted first."""
+    g = _vertex_weighted(heipa.gen_grid(8, 8), 3, 20)
+    part = random_partition(g.n, k, seed, skew)
+    lmax = math.ceil((1 + eps) * int(g.g_weight) / k)
+    (bp, bc, bm), trace = _jet_refine(g, part, k, lmax, False, 0)
+    p, c, mw, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part),
+                                  int(block_weights(g, part, k).max()), trace_cap=4096)
+    got = [(t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
+           for t in tr]
+    assert any(t[2] == 2 for t in trace)
+    assert got == trace
+    assert np.array_equal(p, bp) and (c, mw) == (bc, bm)
