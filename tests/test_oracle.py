@@ -170,7 +170,20 @@ TINY = [
     ("rmat_u", lambda: heipa.gen_rmat(7, 0, 6.0, 0.57, 0.19, 0.19, 6, 0, False)),
     ("star", lambda: star_plus_grid(40, 3, 4)),
     ("isolated", lambda: csr_from_edges(12, [(0, 1), (2, 3)])),
+    # coarse levels: merged edge weights and vertex weights
+    ("rgg_l1", lambda: _coarsened(heipa.gen_rgg(160, 7.0, 8, 10), 1)),
+    ("rmat_u_l2", lambda: _coarsened(heipa.gen_rmat(8, 0, 6.0, 0.57, 0.19, 0.19, 6, 0, False), 2)),
+    ("grid_l2", lambda: _coarsened(heipa.gen_grid(12, 13), 2)),
 ]
+
+
+def _coarsened(g, levels):
+    for _ in range(levels):
+        mp, nc, _ = orc.match(g)
+        cx, ca, cw, cv, rc = orc.contract(g, mp, nc)
+        assert rc == 0
+        g = heipa.HostGraph(cx, ca, cw, cv)
+    return g
 
 
 @pytest.mark.parametrize("name,make", TINY)
