@@ -89,6 +89,36 @@ def test_contract_project(ctx, graph):
     assert np.array_equal(heipa.to_numpy_u32(fine), orc.project(omp, cpart))
 
 
+def _coarsened(g, levels):
+    for _ in range(levels):
+        mp, nc, _ = orc.match(g, 12345)
+        cx, ca, cw, cv, rc = orc.contract(g, mp, nc)
+        assert rc == 0
+        g = heipa.HostGraph(cx, ca, cw, cv)
+    return g
+
+
+COARSE = {
+    "rmat8u_l2": lambda: _coarsened(heipa.gen_rmat(8, 0, 6.0, 0.57, 0.19, 0.19, 6, 0, False), 2),
+    "rmat11u_l1": lambda: _coarsened(GRAPHS["rmat11u"], 1),
+    "rmat11u_l3": lambda: _coarsened(GRAPHS["rmat11u"], 3),
+    "rmat12w_l2": lambda: _coarsened(GRAPHS["rmat12w"], 2),
+    "star_l1": lambda: _coarsened(GRAPHS["star"], 1),
+    "hubs100_l2": lambda: _coarsened(GRAPHS["hubs100"], 2),
+    "rgg14w_l3": lambda: _coarsened(GRAPHS["rgg14w"], 3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(COARSE))
+def test_match_contract_coarse_levels(ctx, name):
+    """Matching and contraction on coarse levels (vertex weights, merged edge
+    weights; the relative pass runs on the R-MAT levels)."""
+    g = COARSE[name]()
+    print(f"coarse level {name} n={g.n}", flush=True)
+    test_match(ctx, (name, g))
+    test_contract_project(ctx, (name, g))
+
+
 def _trace_tuple(t):
     return (t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
 
