@@ -1,0 +1,93 @@
+"""GPU error paths of the C-ABI: malformed graphs, block ids out of range and
+bad arguments are rejected with HEIPA_ERR_INVALID and a message, by the host
+checks or the validation launches that run before any computation, and the
+context stays usable afterwards (the reference exits the process instead,
+host_graph.h:148-151)."""
+import numpy as np
+import pytest
+
+import heipa
+from test_gpu_parity import _dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _raw(xadj, adjncy, adjwgt=None, vwgt=None):
+    return heipa.HostGraph(np.array(xadj, np.uint32), np.array(adjncy, np.uint32),
+                           None if adjwgt is None else np.array(adjwgt, np.int32),
+                           None if vwgt is None else np.array(vwgt, np.int32))
+
+
+def _hub_with_self_loop():
+    """Vertex 0 adjacent to itself and 40 leaves: a row longer than one
+    thread scans, so the wave-wide self-loop scan finds it."""
+    leaves = list(range(1, 41))
+    xadj = [0, 41] + [41 + i for i in range(1, 41)]
+    return _raw(xadj, [0] + leaves + [0] * 40)
+
+
+BAD = {
+    "xadj_not_monotone": (lambda: _raw([0, 3, 2, 4], [1, 2, 0, 1]), "monotone"),
+    "neighbour_out_of_range": (lambda: _raw([0, 1, 2], [1, 7]), "neighbour id >= n"),
+    "zero_edge_weight": (lambda: _raw([0, 1, 2], [1, 0], adjwgt=[0, 0]), "edge weights must be positive"),
+    "negative_vertex_weight": (lambda: _raw([0, 1, 2], [1, 0], vwgt=[1, -2]), "vertex weights must be non-negative"),
+    "self_loop": (lambda: _raw([0, 2, 3], [0, 1, 0]), "self loops"),
+    "hub_self_loop": (_hub_with_self_loop, "self loops"),
+}
+
+
+def _good(ctx):
+    g = heipa.gen_grid(6, 7)
+    part = (np.arange(g.n) % 3).astype(np.uint32)
+    cut = ctx.edge_cut(_dev(g), heipa.from_numpy_u32(part))
+    assert cut > 0
+    return cut
+
+
+@pytest.mark.parametrize("name", sorted(BAD))
+def test_malformed_graph_is_rejected(ctx, name):
+    make, msg = BAD[name]
+    g = make()
+    print(f"malformed graph {name} n={g.n} m={g.m}", flush=True)
+    before = _good(ctx)
+    dg = _dev(g)
+    part = heipa.from_numpy_u32(np.zeros(g.n, np.uint32))
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.edge_cut(dg, part)
+    assert e.value.code == heipa.ERR_INVALID and msg in str(e.value), str(e.value)
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.partition(dg, 2)
+    assert e.value.code == heipa.ERR_INVALID and msg in str(e.value), str(e.value)
+    assert _good(ctx) == before  # the context is still usable
+
+
+def test_block_id_out_of_range_is_rejected(ctx):
+    g = heipa.gen_grid(6, 7)
+    dg = _dev(g)
+    part = (np.arange(g.n) % 3).astype(np.uint32)
+    part[17] = 4
+    pd = heipa.from_numpy_u32(part)
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.jet_refine(dg, pd, 4, 100, False, 0, 0, 0)
+    assert e.value.code == heipa.ERR_INVALID and "block id >= k" in str(e.value)
+    assert np.array_equal(heipa.to_numpy_u32(pd), part)  # untouched
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.block_weights(dg, pd, 4)
+    assert e.value.code == heipa.ERR_INVALID
+    _good(ctx)
+
+
+def test_bad_arguments_are_rejected(ctx):
+    g = heipa.gen_grid(6, 7)
+    dg = _dev(g)
+    pd = heipa.from_numpy_u32(np.zeros(g.n, np.uint32))
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.jet_refine(dg, pd, 0, 100, False, 0, 0, 0)  # k = 0
+    assert e.value.code == heipa.ERR_INVALID
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.partition(dg, 0)
+    assert e.value.code == heipa.ERR_INVALID
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.partition(dg, 4, imbalance=-0.1)
+    assert e.value.code == heipa.ERR_INVALID and "imbalance" in str(e.value)
+    _good(ctx)
