@@ -91,3 +91,24 @@ def test_bad_arguments_are_rejected(ctx):
         ctx.partition(dg, 4, imbalance=-0.1)
     assert e.value.code == heipa.ERR_INVALID and "imbalance" in str(e.value)
     _good(ctx)
+
+
+def test_arena_exhaustion_is_reported_and_recovered(ctx):
+    """A 16 MiB arena cannot hold the coarsening of an RGG with 2^18 vertices
+    (its contraction tables alone take about 25 MB): the solve returns
+    HEIPA_ERR_OOM, leaves the arena as it was, and the same context then
+    solves a small graph with the result a large context gives."""
+    small = heipa.gen_grid(12, 12)
+    ref, _, _ = ctx.partition(_dev(small), 4)
+    ref = heipa.to_numpy_u32(ref).copy()
+    big = _dev(heipa.gen_rgg(1 << 18, 12.0, 3, 10))
+    c = heipa.Context(0, 16 << 20)
+    try:
+        for _ in range(2):
+            with pytest.raises(heipa.HeipaError) as e:
+                c.partition(big, 16)
+            assert e.value.code == heipa.ERR_OOM, str(e.value)
+            p, _, _ = c.partition(_dev(small), 4)
+            assert np.array_equal(heipa.to_numpy_u32(p), ref)
+    finally:
+        c.close()
