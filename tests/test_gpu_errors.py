@@ -112,3 +112,25 @@ def test_arena_exhaustion_is_reported_and_recovered(ctx):
             assert np.array_equal(heipa.to_numpy_u32(p), ref)
     finally:
         c.close()
+
+
+def test_coarse_weight_overflow_is_reported(ctx):
+    """Coarse weights that do not fit the s32 graph format return
+    HEIPA_ERR_OVERFLOW (the reference wraps them): merged edge weights, merged
+    vertex weights, and a whole solve whose first contraction overflows."""
+    from graphs import csr_from_edges
+    big = 2**31 - 1
+    g = csr_from_edges(4, [(0, 2), (1, 2), (0, 3)], [big, big, 1])
+    mp = heipa.from_numpy_u32(np.array([0, 0, 1, 2], np.uint32))  # (0,2) and (1,2) merge
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.contract(_dev(g), mp, 3)
+    assert e.value.code == heipa.ERR_OVERFLOW, str(e.value)
+    g = csr_from_edges(4, [(0, 1), (1, 2), (2, 3)], None, [big, big, 1, 1])
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.contract(_dev(g), mp, 3)  # vertices 0 and 1 merge
+    assert e.value.code == heipa.ERR_OVERFLOW, str(e.value)
+    path = csr_from_edges(40, [(i, i + 1) for i in range(39)], None, [1 << 30] * 40)
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.partition(_dev(path), 2)  # n > 8k: any matched pair weighs 2^31
+    assert e.value.code == heipa.ERR_OVERFLOW, str(e.value)
+    _good(ctx)
