@@ -43,3 +43,23 @@ def test_degenerate_partition(ctx, name, ultra):
     assert np.array_equal(heipa.to_numpy_u32(p)[:g.n], op), name
     if g.n:
         assert ctx.edge_cut(dg, heipa.from_numpy_u32(op)) == ocut
+
+
+MAPPING = ["edgeless100", "edgeless_vw", "empty", "isolated_and_grid", "one_edge", "path_k_above_n", "star_k8"]
+
+
+@pytest.mark.parametrize("hier,dist", [([2, 2], [1, 10]), ([3, 4], [1, 5]), ([1, 2], [2, 3])])
+@pytest.mark.parametrize("name", MAPPING)
+def test_degenerate_im_and_hm(ctx, name, hier, dist):
+    """The IM solver and the HM recursion on the same inputs (the hierarchy
+    product may exceed n; a level of 1 splits nothing)."""
+    g = CASES[name][0]()
+    print(f"degenerate mapping {name} n={g.n} hier={hier}", flush=True)
+    dg = _dev(g)
+    op, oJ, omaxw, _ = orc.im_partition(g, hier, dist, 0.03, 1)
+    p, st, _ = ctx.im_partition(dg, hier, dist, 0.03, 1)
+    assert (st.cut, st.maxw) == (oJ, omaxw), name
+    assert np.array_equal(heipa.to_numpy_u32(p)[:g.n], op), name
+    oh = orc.hm_partition(g, hier, 0.03, 1, False)
+    ph = ctx.hm_partition(dg, hier, 0.03, 1, False)
+    assert np.array_equal(heipa.to_numpy_u32(ph)[:g.n], oh), name
