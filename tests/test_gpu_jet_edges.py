@@ -121,3 +121,33 @@ def test_map_refine_edges(ctx, name, form):
         assert np.array_equal(heipa.to_numpy_u32(pd), op), name
 
     _with_form(ctx, form, run)
+
+
+LARGE_K = heipa.gen_rgg(1 << 14, 10.0, 9, 10)
+
+
+@pytest.mark.parametrize("form", ["multi-graph", "multi-eager"])
+@pytest.mark.parametrize("ultra,level", [(False, 0), (False, 1), (True, 0)])
+@pytest.mark.parametrize("k", [700, 1500, 3000])
+def test_jet_large_k(ctx, k, ultra, level, form):
+    """Wide k: the sweep keeps fewer, wider lane groups per workgroup (one
+    LDS counter array of 12k bytes per group), and with 5-25 vertices per
+    block most iterations rebalance."""
+    g = LARGE_K
+    part0 = (np.arange(g.n, dtype=np.int64) * k // g.n).astype(np.uint32)
+    rnd = random_partition(g.n, k, 11, 0.3)
+    flip = np.random.default_rng(5).random(g.n) < 0.15
+    part0 = np.where(flip, rnd, part0).astype(np.uint32)
+    lmax = math.ceil(1.03 * g.n / k)
+    cut0, maxw0 = orc.edge_cut(g, part0), int(block_weights(g, part0, k).max())
+    op, ocut, omaxw, otr = orc.jet_refine(g, part0, k, lmax, ultra, level, cut0, maxw0, trace_cap=8192)
+    print(f"jet large k={k} ultra={ultra} level={level} form={form}", flush=True)
+
+    def run():
+        pd = heipa.from_numpy_u32(part0)
+        gcut, gmaxw, gtr = ctx.jet_refine(_dev(g), pd, k, lmax, ultra, level, cut0, maxw0, trace_cap=8192)
+        _same(f"k{k}", gtr, otr)
+        assert (gcut, gmaxw) == (ocut, omaxw)
+        assert np.array_equal(heipa.to_numpy_u32(pd), op)
+
+    _with_form(ctx, form, run)
