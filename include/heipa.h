@@ -126,7 +126,10 @@ int heipa_project(heipa_ctx *ctx, const uint32_t *mapping, uint32_t fine_n, cons
 /* ---- refinement */
 /* jet_refine<uvw,uew>(Graph&, Partition&, k, lmax, use_ultra, level, cut, maxw, stack, exec) —
  * src/refinement/jet_label_propagation.h:926-1033.  part is in/out (device); cut and maxw are in/out
- * (the current values on entry, the best partition's on return). */
+ * (the current values on entry, the best partition's on return).  k <= 4093 (the block-connectivity
+ * counters live in LDS; the reference has no such limit): a larger k returns HEIPA_ERR_INVALID.  The
+ * same limit holds for heipa_map_refine, for heipa_partition and heipa_im_partition on graphs with more
+ * than 8k vertices, and for each split (one hierarchy entry) of heipa_hm_partition. */
 int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
                      uint32_t level, int64_t *cut, int64_t *maxw);
 /* same, recording one heipa_jet_trace per iteration (test hook; adds a readback per iteration) */

@@ -134,3 +134,19 @@ def test_coarse_weight_overflow_is_reported(ctx):
         ctx.partition(_dev(path), 2)  # n > 8k: any matched pair weighs 2^31
     assert e.value.code == heipa.ERR_OVERFLOW, str(e.value)
     _good(ctx)
+
+
+def test_jet_k_limit_is_reported(ctx):
+    """The sweep keeps a 12k-byte LDS counter array per lane group (at most
+    48 KiB) and the hub sweep one array plus 4k bytes of block weights (at
+    most 64 KiB), so one Jet call takes k <= 4093; k = 4094 is rejected
+    before any Jet launch, and k = 4093 runs."""
+    g = heipa.gen_grid(6, 7)
+    dg = _dev(g)
+    pd = heipa.from_numpy_u32(np.zeros(g.n, np.uint32))
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.jet_refine(dg, pd, 4094, 100, False, 0, 0, g.n)
+    assert e.value.code == heipa.ERR_INVALID and "k too large" in str(e.value), str(e.value)
+    cut, maxw, _ = ctx.jet_refine(dg, pd, 4093, 1, False, 0, 0, g.n)
+    assert maxw <= g.n and cut == ctx.edge_cut(dg, pd)
+    _good(ctx)
