@@ -95,3 +95,30 @@ def test_pool_rejects_job_larger_than_arena():
         assert j.status == heipa.ERR_OOM and j.slot == -1
     finally:
         pool.close()
+
+
+def test_pool_failed_job_does_not_affect_the_others(ctx):
+    """A malformed graph (a neighbour id >= n, rejected by the validation
+    launches) fails alone with HEIPA_ERR_INVALID; the jobs around it, on the
+    same workers, still equal their alone runs, and the pool accepts more
+    work afterwards."""
+    reqs = _requests()[:4]
+    bad = heipa.HostGraph(np.array([0, 1, 2], np.uint32), np.array([1, 7], np.uint32))
+    bad_dg = heipa.DeviceGraph(bad, "cuda:0")
+    pool = heipa.Pool(2, 512 << 20)
+    try:
+        jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs]
+        jobs.insert(2, heipa.Pool.job(bad_dg, k=2, imbalance=0.03, seed=1))
+        pool.run(jobs)
+        assert jobs[2].status == heipa.ERR_INVALID
+        rest = [j for i, j in enumerate(jobs) if i != 2]
+        for (dg, s), j in zip(reqs, rest):
+            assert j.status == heipa.OK, (s, j.status)
+            assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
+        again = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs[:2]]
+        pool.run(again)
+        for (dg, s), j in zip(reqs[:2], again):
+            assert j.status == heipa.OK
+            assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
+    finally:
+        pool.close()
