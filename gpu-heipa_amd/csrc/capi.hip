@@ -245,9 +245,9 @@ int heipa_project(heipa_ctx *ctx, const uint32_t *mapping, uint32_t fine_n, cons
     });
 }
 
-static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
-                      uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace, uint32_t trace_cap,
-                      uint32_t *trace_len, const int32_t *D = nullptr) {
+static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, uint32_t k,
+                      int64_t lmax, int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
+                      uint32_t trace_cap, uint32_t *trace_len, const int32_t *D = nullptr) {
     return guarded(ctx, [&](Ctx &c) {
         check_graph(g);
         require(cut && maxw && k > 0 && (part || g->n == 0), "null argument");
@@ -259,6 +259,10 @@ static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint
         std::vector<heipa_jet_trace> tr;
         JetResult r =
             jet_refine(c, d, meta, part, k, lmax, ultra != 0, level, *cut, *maxw, trace ? &tr : nullptr, D);
+        // the reference's copy_into carries the best partition's block weights along with its map
+        // (partition.h:192-200), so partition.bweights is current on return
+        if (bweights) block_weights_dev(c, d, part, k, bweights);
+        c.sync();
         *cut = r.cut;
         *maxw = r.maxw;
         if (trace) {
@@ -269,28 +273,28 @@ static int jet_common(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint
     });
 }
 
-int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
-                     uint32_t level, int64_t *cut, int64_t *maxw) {
-    return jet_common(ctx, g, part, k, lmax, ultra, level, cut, maxw, nullptr, 0, nullptr);
+int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, uint32_t k,
+                     int64_t lmax, int ultra, uint32_t level, int64_t *cut, int64_t *maxw) {
+    return jet_common(ctx, g, part, bweights, k, lmax, ultra, level, cut, maxw, nullptr, 0, nullptr);
 }
 
-int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax,
-                            int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
-                            uint32_t trace_cap, uint32_t *trace_len) {
-    return jet_common(ctx, g, part, k, lmax, ultra, level, cut, maxw, trace, trace_cap, trace_len);
+int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, uint32_t k,
+                            int64_t lmax, int ultra, uint32_t level, int64_t *cut, int64_t *maxw,
+                            heipa_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len) {
+    return jet_common(ctx, g, part, bweights, k, lmax, ultra, level, cut, maxw, trace, trace_cap, trace_len);
 }
 
-int heipa_map_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
-                     int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw) {
+int heipa_map_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, const int32_t *D,
+                     uint32_t k, int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw) {
     if (!D) return guarded(ctx, [](Ctx &) { require(false, "null distance matrix"); });
-    return jet_common(ctx, g, part, k, lmax, 0, level, J, maxw, nullptr, 0, nullptr, D);
+    return jet_common(ctx, g, part, bweights, k, lmax, 0, level, J, maxw, nullptr, 0, nullptr, D);
 }
 
-int heipa_map_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
-                            int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw, heipa_jet_trace *trace,
-                            uint32_t trace_cap, uint32_t *trace_len) {
+int heipa_map_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights,
+                            const int32_t *D, uint32_t k, int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw,
+                            heipa_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len) {
     if (!D) return guarded(ctx, [](Ctx &) { require(false, "null distance matrix"); });
-    return jet_common(ctx, g, part, k, lmax, 0, level, J, maxw, trace, trace_cap, trace_len, D);
+    return jet_common(ctx, g, part, bweights, k, lmax, 0, level, J, maxw, trace, trace_cap, trace_len, D);
 }
 
 static void fill_stats(heipa_solve_stats *stats, const SolveResult &r) {
@@ -323,6 +327,8 @@ int heipa_im_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hie
             k *= x;
             require(k <= (1u << 16), "hierarchy product too large");
         }
+        // the mapping Jet runs on every level above the coarsest, which exists once n > 8k
+        require(g->n <= 8 * k || k <= JET_MAX_K, "hierarchy product too large for the mapping refinement");
         DGraph d = to_dgraph(g);
         validate_graph(c, d);
         std::vector<s32> Dh(k * k);
@@ -376,8 +382,14 @@ int heipa_hm_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hie
     return guarded(ctx, [&](Ctx &c) {
         check_graph(g);
         require(hier && l > 0 && (part || g->n == 0), "bad argument");
+        require(imbalance >= 0.0, "imbalance must be non-negative");
         std::vector<u32> h(hier, hier + l);
-        for (u32 x: h) require(x > 0, "hierarchy entries must be positive");
+        u64 prod = 1;
+        for (u32 x: h) {
+            require(x > 0, "hierarchy entries must be positive");
+            prod *= x;
+            require(prod <= (1u << 16), "hierarchy product too large");
+        }
         DGraph d = to_dgraph(g);
         validate_graph(c, d);
         hm_partition(c, d, h, imbalance, seed, ultra != 0, part);
@@ -405,7 +417,13 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         else if (option == HEIPA_OPT_PROF_SAMPLE) c.prof_sample = value < 1 ? 1u : (u32) std::min<uint64_t>(value, 1u << 30);
         else if (option == HEIPA_OPT_BUCKET_COPIES) c.bucket_copies = (u32) std::min<uint64_t>(value, 32);
         else if (option == HEIPA_OPT_BLOCK_LDS) c.block_lds = value != 0;
-        else if (option == HEIPA_OPT_HM_WORKERS) c.hm_workers = (u32) std::min<uint64_t>(value, 64);
+        else if (option == HEIPA_OPT_HM_WORKERS) {
+            c.hm_workers = (u32) std::min<uint64_t>(value, 64);
+            ctx_trim_workers(c, c.hm_workers > 1 ? c.hm_workers : 0);  // frees the arenas of the ones dropped
+        } else if (option == HEIPA_OPT_HM_WORKER_BYTES) {
+            c.hm_worker_bytes = value;
+            ctx_trim_workers(c, 0);  // reopened at the next split with the new size
+        }
         else require(false, "unknown option");
     });
 }

@@ -159,6 +159,7 @@ struct Ctx {
     u32 bucket_copies = 0;   // HEIPA_OPT_BUCKET_COPIES (0: automatic)
     bool block_lds = true;   // HEIPA_OPT_BLOCK_LDS
     u32 hm_workers = 8;      // HEIPA_OPT_HM_WORKERS
+    u64 hm_worker_bytes = 0; // HEIPA_OPT_HM_WORKER_BYTES (0: the estimate)
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;
@@ -276,6 +277,15 @@ inline void ctx_close(Ctx &c) {
     c.stream = nullptr;
 }
 
+// Closes the cached worker contexts beyond the first `keep` (their arenas go
+// back to the device).
+inline void ctx_trim_workers(Ctx &c, size_t keep) {
+    for (size_t i = keep; i < c.workers.size(); ++i)
+        if (c.workers[i]) ctx_close(*c.workers[i]);
+    if (c.workers.size() > keep) c.workers.resize(keep);
+    (void) hipSetDevice(c.device);
+}
+
 // The tuning options of `from` (results never depend on them) and its
 // profiling settings, for a worker that runs part of its call.
 inline void ctx_copy_settings(Ctx &to, const Ctx &from) {
@@ -285,6 +295,7 @@ inline void ctx_copy_settings(Ctx &to, const Ctx &from) {
     to.bucket_copies = from.bucket_copies;
     to.block_lds = from.block_lds;
     to.hm_workers = from.hm_workers;
+    to.hm_worker_bytes = from.hm_worker_bytes;
     to.prof_on = from.prof_on;
     to.prof_filter = from.prof_filter;
     to.prof_sample = from.prof_sample;

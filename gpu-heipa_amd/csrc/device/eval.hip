@@ -174,6 +174,17 @@ __global__ __launch_bounds__(256) void block_weights_kernel(u32 n, const u32 *__
         if (hist[i]) atomicAdd(&bw[i], hist[i]);
 }
 
+// block weights for k beyond the LDS histogram: global 64-bit atomics
+template<bool UVW>
+__global__ __launch_bounds__(256) void block_weights_global_kernel(u32 n, const u32 *__restrict__ part,
+                                                                   const s32 *__restrict__ vw,
+                                                                   unsigned long long *bw) {
+    for (u64 u = (u64) blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (u64) gridDim.x * blockDim.x) {
+        const s64 w = UVW ? 1 : vw[u];
+        if (w) atomicAdd(&bw[part[u]], (unsigned long long) w);
+    }
+}
+
 // input validation: xadj monotone within [0, m], neighbour ids < n, edge
 // weights > 0, vertex weights >= 0, no self loops; flags bit 0 = bad offsets,
 // 1 = bad id, 2 = bad edge weight, 3 = bad vertex weight, 4 = self loop.
@@ -290,9 +301,17 @@ s64 comm_cost(Ctx &c, const DGraph &g, const GraphMeta &meta, const u32 *part, c
 
 void block_weights_dev(Ctx &c, const DGraph &g, const u32 *part, u32 k, s64 *bw_dev) {
     fill<s64>(c, bw_dev, k, 0);
-    const size_t shmem = sizeof(unsigned long long) * k;
-    require(shmem <= 64 * 1024, "k too large for the block-weight histogram");
     const double bytes = 4.0 * g.n * (g.vw ? 2 : 1);
+    const size_t shmem = sizeof(unsigned long long) * k;
+    if (shmem > 64 * 1024) {  // no LDS histogram: one global atomic per vertex
+        if (g.vw)
+            launch(c, "block_weights", bytes, dim3(grid_for(c, g.n, 256)), dim3(256), 0,
+                   block_weights_global_kernel<false>, g.n, part, g.vw, (unsigned long long *) bw_dev);
+        else
+            launch(c, "block_weights", bytes, dim3(grid_for(c, g.n, 256)), dim3(256), 0,
+                   block_weights_global_kernel<true>, g.n, part, g.vw, (unsigned long long *) bw_dev);
+        return;
+    }
     if (g.vw)
         launch(c, "block_weights", bytes, dim3(grid_for(c, g.n, 256 * 8)), dim3(256), shmem,
                block_weights_kernel<false>, g.n, part, g.vw, k, (unsigned long long *) bw_dev);

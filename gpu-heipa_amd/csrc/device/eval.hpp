@@ -29,6 +29,9 @@ CoarseGraph contract(Ctx &c, const DGraph &fine, const GraphMeta &meta, const u3
 void project(Ctx &c, const u32 *mapping, u32 fine_n, const u32 *coarse_part, u32 *fine_part);
 
 // ---- Jet refinement (jet.hip)
+// Largest k one Jet call takes: the sweep keeps one LDS counter per block for
+// each lane group (12k bytes against a 48 KiB budget).
+constexpr u32 JET_MAX_K = 4093;
 struct JetResult {
     s64 cut = 0, maxw = 0;
     u32 iterations = 0;

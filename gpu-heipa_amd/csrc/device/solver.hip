@@ -283,6 +283,7 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
     const size_t nw = std::min<size_t>(kids.size(), c.hm_workers);
     u64 need = 0;
     for (const Kid &kd: kids) need = std::max(need, hm_child_bytes(kd.g, p, level - 1));
+    if (c.hm_worker_bytes) need = c.hm_worker_bytes;
     try {
         if (c.workers.size() < nw) c.workers.resize(nw);
         for (size_t i = 0; i < nw; ++i) {
@@ -305,6 +306,7 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
     // workers take the children in turn: kid i runs on worker i mod nw,
     // each worker's children one after another on its own stream
     std::vector<std::exception_ptr> errors(nw);
+    std::vector<u8> done(kids.size(), 0);
     auto run = [&](size_t wi) {
         Ctx &w = *c.workers[wi];
         const size_t fd = w.arena.front_depth(), bd = w.arena.back_depth();
@@ -314,8 +316,9 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
             for (size_t i = wi; i < kids.size(); i += nw) {
                 std::vector<u32> ident{kids[i].id};
                 hm_rec(w, kids[i].g, kids[i].n_to_o, level - 1, ident, p, global);
+                w.sync();
+                done[i] = 1;
             }
-            w.sync();
         } catch (...) {
             errors[wi] = std::current_exception();
             // the worker is reused by later calls: leave its arena as it was
@@ -334,8 +337,24 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
     }
     for (auto &t: threads) t.join();
     for (size_t wi = 0; wi < nw; ++wi) ctx_merge_prof(c, *c.workers[wi]);
-    for (auto &e: errors)
-        if (e) std::rethrow_exception(e);
+    HIP_CHECK(hipSetDevice(c.device));
+    // A worker whose arena (sized by the estimate) ran out hands its
+    // unfinished children back: they run one after another on c, whose arena
+    // is the caller's.  A child solved again rewrites the same output entries
+    // with the same values.  Any other error is the call's.
+    for (size_t wi = 0; wi < nw; ++wi) {
+        if (!errors[wi]) continue;
+        try {
+            std::rethrow_exception(errors[wi]);
+        } catch (const Error &e) {
+            if (e.code != HEIPA_ERR_OOM) throw;
+        }
+        for (size_t i = wi; i < kids.size(); i += nw) {
+            if (done[i]) continue;
+            std::vector<u32> ident{kids[i].id};
+            hm_rec(c, kids[i].g, kids[i].n_to_o, level - 1, ident, p, global);
+        }
+    }
     return true;
 }
 

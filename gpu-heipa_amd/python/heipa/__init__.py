@@ -27,6 +27,7 @@ OPT_PROF_SAMPLE = 4
 OPT_BUCKET_COPIES = 5
 OPT_BLOCK_LDS = 6
 OPT_HM_WORKERS = 7
+OPT_HM_WORKER_BYTES = 8
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -121,14 +122,14 @@ def lib():
             "heipa_match": (c_int, [vp, GP, c_u32, vp, u32p, ctypes.POINTER(MatchStats)]),
             "heipa_contract": (c_int, [vp, GP, vp, c_u32, GP]),
             "heipa_project": (c_int, [vp, vp, c_u32, vp, vp]),
-            "heipa_jet_refine": (c_int, [vp, GP, vp, c_u32, ctypes.c_int64, c_int, c_u32, s64p, s64p]),
-            "heipa_jet_refine_traced": (c_int, [vp, GP, vp, c_u32, ctypes.c_int64, c_int, c_u32, s64p, s64p,
+            "heipa_jet_refine": (c_int, [vp, GP, vp, vp, c_u32, ctypes.c_int64, c_int, c_u32, s64p, s64p]),
+            "heipa_jet_refine_traced": (c_int, [vp, GP, vp, vp, c_u32, ctypes.c_int64, c_int, c_u32, s64p, s64p,
                                                 ctypes.POINTER(JetTrace), c_u32, u32p]),
             "heipa_partition": (c_int, [vp, GP, c_u32, c_dbl, c_u64, c_int, vp, ctypes.POINTER(SolveStats),
                                         ctypes.POINTER(LevelTrace), c_u32]),
             "heipa_hm_partition": (c_int, [vp, GP, u32p, c_u32, c_dbl, c_u64, c_int, vp]),
-            "heipa_map_refine": (c_int, [vp, GP, vp, vp, c_u32, ctypes.c_int64, c_u32, s64p, s64p]),
-            "heipa_map_refine_traced": (c_int, [vp, GP, vp, vp, c_u32, ctypes.c_int64, c_u32, s64p, s64p,
+            "heipa_map_refine": (c_int, [vp, GP, vp, vp, vp, c_u32, ctypes.c_int64, c_u32, s64p, s64p]),
+            "heipa_map_refine_traced": (c_int, [vp, GP, vp, vp, vp, c_u32, ctypes.c_int64, c_u32, s64p, s64p,
                                                 ctypes.POINTER(JetTrace), c_u32, u32p]),
             "heipa_im_partition": (c_int, [vp, GP, u32p, s32p, c_u32, c_dbl, c_u64, vp, ctypes.POINTER(SolveStats),
                                            ctypes.POINTER(LevelTrace), c_u32]),
@@ -432,18 +433,22 @@ class Context:
                                                           fine.data_ptr()))
         return fine[:fine_n]
 
-    def jet_refine(self, g: DeviceGraph, part, k, lmax, ultra, level, cut, maxw, trace_cap=0):
+    def jet_refine(self, g: DeviceGraph, part, k, lmax, ultra, level, cut, maxw, trace_cap=0, bweights=None):
+        """jet_refine on the device; bweights (optional device int64 tensor of k entries) receives the
+        returned partition's block weights.  Returns (cut, maxw, trace)."""
         self._sync()
         c, mw = ctypes.c_int64(cut), ctypes.c_int64(maxw)
+        bw = bweights.data_ptr() if bweights is not None else None
         if trace_cap:
             tr = (JetTrace * trace_cap)()
             tl = ctypes.c_uint32()
             self._check("heipa_jet_refine_traced", self.L.heipa_jet_refine_traced(
-                self.h, ctypes.byref(g.desc), part.data_ptr(), k, lmax, int(ultra), level, ctypes.byref(c),
+                self.h, ctypes.byref(g.desc), part.data_ptr(), bw, k, lmax, int(ultra), level, ctypes.byref(c),
                 ctypes.byref(mw), tr, trace_cap, ctypes.byref(tl)))
             return c.value, mw.value, [tr[i] for i in range(min(tl.value, trace_cap))]
-        self._check("heipa_jet_refine", self.L.heipa_jet_refine(self.h, ctypes.byref(g.desc), part.data_ptr(), k, lmax,
-                                                                int(ultra), level, ctypes.byref(c), ctypes.byref(mw)))
+        self._check("heipa_jet_refine", self.L.heipa_jet_refine(self.h, ctypes.byref(g.desc), part.data_ptr(), bw, k,
+                                                                lmax, int(ultra), level, ctypes.byref(c),
+                                                                ctypes.byref(mw)))
         return c.value, mw.value, None
 
     def partition(self, g: DeviceGraph, k, imbalance=0.03, seed=1, ultra=False, part=None, level_cap=0):
@@ -459,18 +464,20 @@ class Context:
         levels = [lv[i] for i in range(min(st.levels, level_cap))] if level_cap else None
         return part[:g.n], st, levels
 
-    def map_refine(self, g: DeviceGraph, part, D, k, lmax, level, J, maxw, trace_cap=0):
-        """ProMap_jet_refine on the device: D is a device int32 k x k tensor; returns (J, maxw, trace)."""
+    def map_refine(self, g: DeviceGraph, part, D, k, lmax, level, J, maxw, trace_cap=0, bweights=None):
+        """ProMap_jet_refine on the device: D is a device int32 k x k tensor; bweights as for jet_refine;
+        returns (J, maxw, trace)."""
         self._sync()
         j, mw = ctypes.c_int64(J), ctypes.c_int64(maxw)
+        bw = bweights.data_ptr() if bweights is not None else None
         if trace_cap:
             tr = (JetTrace * trace_cap)()
             tl = ctypes.c_uint32()
             self._check("heipa_map_refine_traced", self.L.heipa_map_refine_traced(
-                self.h, ctypes.byref(g.desc), part.data_ptr(), D.data_ptr(), k, lmax, level, ctypes.byref(j),
+                self.h, ctypes.byref(g.desc), part.data_ptr(), bw, D.data_ptr(), k, lmax, level, ctypes.byref(j),
                 ctypes.byref(mw), tr, trace_cap, ctypes.byref(tl)))
             return j.value, mw.value, [tr[i] for i in range(min(tl.value, trace_cap))]
-        self._check("heipa_map_refine", self.L.heipa_map_refine(self.h, ctypes.byref(g.desc), part.data_ptr(),
+        self._check("heipa_map_refine", self.L.heipa_map_refine(self.h, ctypes.byref(g.desc), part.data_ptr(), bw,
                                                                 D.data_ptr(), k, lmax, level, ctypes.byref(j),
                                                                 ctypes.byref(mw)))
         return j.value, mw.value, None

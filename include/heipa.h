@@ -105,7 +105,8 @@ int heipa_comm_cost(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, 
 /* initialize_distance_oracle_matrix(k, hierarchy, distance, ...) —
  * src/distance_oracles/distance_oracle_matrix.h:41-90; host output, k = prod(hier) */
 int heipa_distance_matrix(const uint32_t *hier, const int32_t *dist, uint32_t l, int32_t *D_host);
-/* recalculate_weights + max_weight — src/datastructures/partition.h:107-134; bw_host has k entries */
+/* recalculate_weights + max_weight — src/datastructures/partition.h:107-134; bw_host has k entries (any k:
+ * up to 8192 blocks are summed in LDS, more with global atomics) */
 int heipa_block_weights(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, uint32_t k, int64_t *bw_host,
                         int64_t *maxw);
 
@@ -126,31 +127,34 @@ int heipa_project(heipa_ctx *ctx, const uint32_t *mapping, uint32_t fine_n, cons
 /* ---- refinement */
 /* jet_refine<uvw,uew>(Graph&, Partition&, k, lmax, use_ultra, level, cut, maxw, stack, exec) —
  * src/refinement/jet_label_propagation.h:926-1033.  part is in/out (device); cut and maxw are in/out
- * (the current values on entry, the best partition's on return).  k <= 4093 (the block-connectivity
- * counters live in LDS; the reference has no such limit): a larger k returns HEIPA_ERR_INVALID.  The
- * same limit holds for heipa_map_refine, for heipa_partition and heipa_im_partition on graphs with more
- * than 8k vertices, and for each split (one hierarchy entry) of heipa_hm_partition. */
-int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax, int ultra,
-                     uint32_t level, int64_t *cut, int64_t *maxw);
+ * (the current values on entry, the best partition's on return).  bweights (device, k entries, may be
+ * NULL) receives the block weights of the returned partition, as the reference's partition.bweights
+ * (copy_into, src/datastructures/partition.h:192-200).  k <= 4093 (the block-connectivity counters live
+ * in LDS; the reference has no such limit): a larger k returns HEIPA_ERR_INVALID.  The same limit holds
+ * for heipa_map_refine, for heipa_partition and heipa_im_partition on graphs with more than 8k vertices,
+ * and for each split (one hierarchy entry) of heipa_hm_partition. */
+int heipa_jet_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, uint32_t k,
+                     int64_t lmax, int ultra, uint32_t level, int64_t *cut, int64_t *maxw);
 /* same, recording one heipa_jet_trace per iteration (test hook; adds a readback per iteration) */
-int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, uint32_t k, int64_t lmax,
-                            int ultra, uint32_t level, int64_t *cut, int64_t *maxw, heipa_jet_trace *trace,
-                            uint32_t trace_cap, uint32_t *trace_len);
+int heipa_jet_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, uint32_t k,
+                            int64_t lmax, int ultra, uint32_t level, int64_t *cut, int64_t *maxw,
+                            heipa_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len);
 
 /* ---- IM process mapping (mapping objective J = sum over directed edges w(u,v) * D(part(u), part(v))) */
 /* ProMap_jet_refine<uvw,uew> — src/refinement/promap_jet_label_propagation.h:1219-1311.  D: device, k x k
  * row-major distances (heipa_distance_matrix, uploaded).  part in/out (device); J and maxw in/out (current
- * values on entry, the best partition's on return).  level is accepted for parity with the reference's
- * signature; the mapping LP reads no filter ratio. */
-int heipa_map_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
-                     int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw);
+ * values on entry, the best partition's on return); bweights as for heipa_jet_refine.  level is accepted
+ * for parity with the reference's signature; the mapping LP reads no filter ratio. */
+int heipa_map_refine(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights, const int32_t *D,
+                     uint32_t k, int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw);
 /* same, recording one heipa_jet_trace per iteration that moved vertices (cut fields hold J) */
-int heipa_map_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, const int32_t *D, uint32_t k,
-                            int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw, heipa_jet_trace *trace,
-                            uint32_t trace_cap, uint32_t *trace_len);
+int heipa_map_refine_traced(heipa_ctx *ctx, const heipa_graph *g, uint32_t *part, int64_t *bweights,
+                            const int32_t *D, uint32_t k, int64_t lmax, uint32_t level, int64_t *J, int64_t *maxw,
+                            heipa_jet_trace *trace, uint32_t trace_cap, uint32_t *trace_len);
 /* ProMapSolver<DistanceOracleMatrix>(config).solve(host_g) with --config IM — src/datastructures/
  * promap_solver.h:146-347: coarsening, global multisection, J-driven Jet per level.  part: device.
- * stats->cut / initial_cut and the level trace cut fields hold J.  prod(hier) <= 65536. */
+ * stats->cut / initial_cut and the level trace cut fields hold J.  prod(hier) <= 65536, and <= 4093 when
+ * n > 8 prod(hier) (the mapping refinement runs then); checked before the k x k matrix is built. */
 int heipa_im_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hier, const int32_t *dist, uint32_t l,
                        double imbalance, uint64_t seed, uint32_t *part, heipa_solve_stats *stats,
                        heipa_level_trace *levels, uint32_t level_cap);
@@ -161,7 +165,9 @@ int heipa_im_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hie
 int heipa_partition(heipa_ctx *ctx, const heipa_graph *g, uint32_t k, double imbalance, uint64_t seed, int ultra,
                     uint32_t *part, heipa_solve_stats *stats, heipa_level_trace *levels, uint32_t level_cap);
 /* ProMapSolver<DistanceOracleMatrix>(config).solve_multisection(host_g) —
- * src/datastructures/promap_solver.h:238-241 -> src/initial_partitioning/hierarchical_multisection.h:237-307 */
+ * src/datastructures/promap_solver.h:238-241 -> src/initial_partitioning/hierarchical_multisection.h:237-307.
+ * imbalance >= 0; prod(hier) <= 65536.  The subtrees below the first split run on worker contexts
+ * (HEIPA_OPT_HM_WORKERS); a worker whose arena runs out hands its unfinished subtrees back to ctx. */
 int heipa_hm_partition(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *hier, uint32_t l, double imbalance,
                        uint64_t seed, int ultra, uint32_t *part);
 /* determine_adaptive_imbalance — src/initial_partitioning/global_multisection.h:39-48 */
@@ -241,9 +247,15 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
 #define HEIPA_OPT_BLOCK_LDS 6
 /* HEIPA_OPT_HM_WORKERS: worker contexts (each its own stream, arena and host thread, kept by the
  * context between calls) on which heipa_hm_partition solves the subtrees below its first split at
- * the same time (default 8, at most 64; 0 or 1: one after another on the context's stream).  Results
- * do not depend on it. */
+ * the same time (default 8, at most 64; 0 or 1: one after another on the context's stream).  Setting
+ * it closes the cached workers beyond the new count, which frees their arenas.  Results do not
+ * depend on it. */
 #define HEIPA_OPT_HM_WORKERS 7
+/* HEIPA_OPT_HM_WORKER_BYTES: arena bytes of each HM worker context (default 0: an estimate from the
+ * largest child's n and m).  A worker whose arena runs out hands its unfinished subtrees back to the
+ * calling context, so a small value still gives the same result.  Setting it closes the cached
+ * workers. */
+#define HEIPA_OPT_HM_WORKER_BYTES 8
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 
 /* ---- per-kernel timing (HIP events on the context stream) */

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import heipa
+import heipa_oracle as orc
 from test_gpu_parity import _dev
 
 pytestmark = pytest.mark.gpu
@@ -90,7 +91,43 @@ def test_bad_arguments_are_rejected(ctx):
     with pytest.raises(heipa.HeipaError) as e:
         ctx.partition(dg, 4, imbalance=-0.1)
     assert e.value.code == heipa.ERR_INVALID and "imbalance" in str(e.value)
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.hm_partition(dg, [2, 2], -0.1, 1, False)
+    assert e.value.code == heipa.ERR_INVALID and "imbalance" in str(e.value)
+    with pytest.raises(heipa.HeipaError) as e:  # product 2^32 wraps u32 arithmetic
+        ctx.hm_partition(dg, [65536, 65536], 0.03, 1, False)
+    assert e.value.code == heipa.ERR_INVALID and "product" in str(e.value)
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.hm_partition(dg, [2, 32769], 0.03, 1, False)
+    assert e.value.code == heipa.ERR_INVALID and "product" in str(e.value)
     _good(ctx)
+
+
+def test_im_rejects_a_product_beyond_the_jet_limit_before_the_matrix(ctx):
+    """An IM call whose mapping refinement would exceed the Jet k limit is
+    rejected before the k x k distance matrix is built."""
+    g = heipa.gen_rgg(40000, 8.0, 3, 10)
+    with pytest.raises(heipa.HeipaError) as e:
+        ctx.im_partition(_dev(g), [4, 1024], [1, 10])
+    assert e.value.code == heipa.ERR_INVALID and "too large" in str(e.value)
+    _good(ctx)
+
+
+def test_block_weights_beyond_the_lds_histogram(ctx):
+    """k above 8192 sums block weights with global atomics: heipa_block_weights,
+    a whole partition (n <= 8k, so no Jet) and a one-level HM pool job."""
+    g = heipa.gen_rgg(30000, 8.0, 5, 10)
+    dg = _dev(g)
+    rng = np.random.default_rng(3)
+    part = rng.integers(0, 9000, g.n).astype(np.uint32)
+    bw, mw = ctx.block_weights(dg, heipa.from_numpy_u32(part), 9000)
+    ref = np.bincount(part, minlength=9000)
+    assert np.array_equal(np.asarray(bw), ref) and mw == ref.max()
+    gp, st, _ = ctx.partition(dg, 8193, 0.03, 1, False)
+    op, cut, maxw, _ = orc.partition(g, 8193, 0.03, 1, False)
+    assert np.array_equal(heipa.to_numpy_u32(gp), op) and st.cut == cut and st.maxw == maxw
+    hp = heipa.to_numpy_u32(ctx.hm_partition(dg, [8193], 0.03, 1, False))
+    assert np.array_equal(hp, orc.hm_partition(g, [8193], 0.03, 1, False))
 
 
 def test_arena_exhaustion_is_reported_and_recovered(ctx):

@@ -117,6 +117,6 @@ def test_map_refine_rejects_null_distances(ctx):
     pd = heipa.from_numpy_u32(np.zeros(g.n, np.uint32))
     with pytest.raises(heipa.HeipaError):
         ctx._check("heipa_map_refine", ctx.L.heipa_map_refine(ctx.h, heipa.ctypes.byref(dg.desc), pd.data_ptr(),
-                                                               None, 4, 100, 0, heipa.ctypes.byref(
+                                                               None, None, 4, 100, 0, heipa.ctypes.byref(
                                                                    heipa.ctypes.c_int64(0)),
                                                                heipa.ctypes.byref(heipa.ctypes.c_int64(0))))

@@ -140,7 +140,9 @@ def test_jet(ctx, graph, ultra, level, skew, k):
     maxw0 = int(block_weights(g, part0, k).max())
     op, ocut, omaxw, otr = orc.jet_refine(g, part0, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
     pd = heipa.from_numpy_u32(part0)
-    gcut, gmaxw, gtr = ctx.jet_refine(dg, pd, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096)
+    import torch
+    bwd = torch.full((k,), -1, dtype=torch.int64, device="cuda:0")
+    gcut, gmaxw, gtr = ctx.jet_refine(dg, pd, k, lmax, ultra, level, cut0, maxw0, trace_cap=4096, bweights=bwd)
     assert len(gtr) == len(otr), (name, len(gtr), len(otr))
     for i, (a, b) in enumerate(zip(gtr, otr)):
         assert _trace_tuple(a) == _trace_tuple(b), (name, i, _trace_tuple(a), _trace_tuple(b))
@@ -148,6 +150,9 @@ def test_jet(ctx, graph, ultra, level, skew, k):
     gp = heipa.to_numpy_u32(pd)
     assert np.array_equal(gp, op), name
     assert orc.edge_cut(g, gp) == gcut
+    # the returned partition's block weights (partition.bweights after copy_into)
+    bw = bwd.cpu().numpy()
+    assert np.array_equal(bw, block_weights(g, op, k)) and bw.max() == gmaxw, name
 
 
 @pytest.mark.parametrize("graphs", [1, 0])
@@ -247,6 +252,17 @@ def test_hm(ctx, hier):
     D = orc.distance_matrix(hier, [1, 10, 100])
     Dd = heipa.from_numpy_u32(D.reshape(-1).view(np.uint32))
     assert ctx.comm_cost(dg, heipa.from_numpy_u32(gp), Dd, k) == orc.comm_cost(g, op, D)
+
+
+def test_hm_worker_out_of_memory_falls_back(ctx):
+    """Worker arenas of 1 MiB cannot hold a child's solve: each worker hands
+    its unfinished subtrees back to the calling context, and the mapping is the
+    oracle's."""
+    ctx.set_option(heipa.OPT_HM_WORKER_BYTES, 1 << 20)
+    try:
+        test_hm(ctx, [4, 8, 6])
+    finally:
+        ctx.set_option(heipa.OPT_HM_WORKER_BYTES, 0)
 
 
 @pytest.mark.parametrize("workers", [0, 2])
