@@ -116,6 +116,7 @@ def lib():
             "heipa_download": (c_int, [vp, vp, vp, c_u64]),
             "heipa_upload": (c_int, [vp, vp, vp, c_u64]),
             "heipa_edge_cut": (c_int, [vp, GP, vp, s64p]),
+            "heipa_edge_cut_noalloc": (c_int, [GP, vp, vp, vp]),
             "heipa_comm_cost": (c_int, [vp, GP, vp, vp, c_u32, s64p]),
             "heipa_distance_matrix": (c_int, [u32p, s32p, c_u32, s32p]),
             "heipa_block_weights": (c_int, [vp, GP, vp, c_u32, s64p, s64p]),

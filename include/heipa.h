@@ -102,6 +102,16 @@ int heipa_edge_cut(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, i
  * (64-bit, not halved); D is a device k*k s32 matrix */
 int heipa_comm_cost(heipa_ctx *ctx, const heipa_graph *g, const uint32_t *part, const int32_t *D, uint32_t k,
                     int64_t *J);
+/* No counterpart in the reference: heipa_edge_cut for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It takes
+ * no context and allocates nothing: its partial sums live in a static device array of 32 KiB
+ * (2048 workgroup slots) that each device holds until the process ends; every input size fits it
+ * (the launch uses at most 2048 workgroups).  Asynchronous on `stream` (a hipStream_t of the
+ * current device, NULL for its default stream): *cut_dev (device int64) receives the cut, or -1
+ * when the graph is malformed (where heipa_edge_cut returns HEIPA_ERR_INVALID).  Returns
+ * HEIPA_ERR_INVALID for null arguments, HEIPA_ERR_DEVICE if a launch fails.  Calls that overlap on
+ * one device share the array and race: order them on one stream. */
+int heipa_edge_cut_noalloc(const heipa_graph *g, const uint32_t *part, int64_t *cut_dev, void *stream);
 /* initialize_distance_oracle_matrix(k, hierarchy, distance, ...) —
  * src/distance_oracles/distance_oracle_matrix.h:41-90; host output, k = prod(hier) */
 int heipa_distance_matrix(const uint32_t *hier, const int32_t *dist, uint32_t l, int32_t *D_host);
