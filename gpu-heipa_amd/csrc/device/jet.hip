@@ -9,7 +9,13 @@
 //     dense per-group k-counter array with 64-bit LDS atomics; the first adder
 //     of a block appends it to a per-group touched list.  The reference's
 //     dest_cache/gain_cache (jet:194-237) decide which vertices are rebuilt.
-//     Ties between blocks go to the smallest block id.
+//     Ties between blocks go to the smallest block id.  What a BlockConn
+//     row holds beyond the positive connections is tracked for the vertices
+//     whose row capacity is k (degree >= k, "wide"): the row's size and the
+//     set of its valid entries, which keeps a block whose connection fell to
+//     0 while the size is k (block_conn.h:379) until update_large rebuilds
+//     the row.  Strong rebalance counts such entries in its mean (jet:418-428)
+//     and the mapping LP takes them as candidates (promap:237-246).
 //   * the cut change of a move set is computed exactly in 64 bits from the old
 //     and new blocks of each moved vertex and its neighbours; it equals the
 //     reference's (c1 + c2) / 2 (jet:763-923).
@@ -138,6 +144,7 @@ struct JetCtl {
     u32 ri, it, bal;
     u32 iters;        // iterations that changed the partition
     u32 n_trace;
+    u32 last_moves;   // length of the last iteration's move list (update_large when > n / 10)
 };
 static_assert(sizeof(CtlHead) == 32 && offsetof(JetCtl, st) == sizeof(CtlHead), "control block head layout");
 
@@ -174,6 +181,12 @@ struct JetDev {
     s64 *evict_start;   // k + 1
     s64 *counter;       // strong-rebalance overflow position
     const s32 *D;       // k x k distances: mapping objective J (ProMap); null: edge cut
+    // BlockConn rows of the wide vertices (degree >= k); rflag null when the level has none
+    u8 *rflag;          // n: bit 0 wide, bit 1 a neighbour is in this iteration's move list
+    u32 *wid;           // n: rank among the wide vertices
+    u32 *wsz;           // per wide vertex: the row's size (BlockConn sizes)
+    u32 *wV, *wP;       // per wide vertex, W words each: blocks of its valid entries / of positive ones
+    u32 W;              // words per block mask, ceil(k / 32)
 };
 
 // Per-vertex staleness of the caches, the n bytes after the locks: bit 0
@@ -207,6 +220,20 @@ __device__ __forceinline__ s64 max_block_weight(const JetDev &d, bool strong) {
     const s64 a = (s64) ((double) d.lmax * 0.99);
     if (strong) return d.opt + 1 > a ? d.opt + 1 : a;
     return a < d.lmax - 100 ? d.lmax - 100 : a;
+}
+
+// a neighbour of a vertex in the move list: its row is updated after the
+// iteration (benign race: every writer stores the same value)
+__device__ __forceinline__ void row_touch(const JetDev &d, u32 v) {
+    if (d.rflag && (d.rflag[v] & 1u)) d.rflag[v] = 3;
+}
+
+// the wide vertex u's row holds k entries: its valid entries include the
+// blocks whose connection fell to 0 since the row filled up
+__device__ __forceinline__ bool row_full(const JetDev &d, u32 u, u32 &r) {
+    if (!d.rflag || !(d.rflag[u] & 1u)) return false;
+    r = d.wid[u];
+    return d.wsz[r] == d.k;
 }
 
 // orders the LDS traffic of one wave (lane groups never span waves)
@@ -434,6 +461,27 @@ __device__ __forceinline__ bool lp_eval(const JetDev &d, const GroupConn &gc, u3
             Best x{jo - jb, b};
             if (best_better(x, best)) best = x;
         }
+        // blocks of weight-0 entries of a full row are candidates as well
+        // (promap:237-246 walks every valid entry)
+        u32 r = 0;
+        if (active && row_full(d, u, r)) {
+            for (u32 i = lane; i < d.W; i += G) {
+                u32 z = d.wV[(u64) r * d.W + i] & ~d.wP[(u64) r * d.W + i];
+                if (own_b >> 5 == i) z &= ~(1u << (own_b & 31));
+                while (z) {
+                    const u32 b = i * 32 + (u32) __builtin_ctz(z);
+                    z &= z - 1;
+                    const s32 *Db = d.D + (u64) b * d.k;
+                    s64 jb = 0;
+                    for (u32 j = 0; j < T; ++j) {
+                        const u32 c = gc.touched[j];
+                        jb += (s64) gc.cnt[c] * (s64) Db[c];
+                    }
+                    Best x{jo - jb, b};
+                    if (best_better(x, best)) best = x;
+                }
+            }
+        }
     } else {
         for (u32 i = lane; i < T; i += G) {
             const u32 b = gc.touched[i];
@@ -568,6 +616,19 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
             if (d.bw[b] < mbw) {
                 sum += (s64) gc.cnt[b];
                 count += 1;
+            }
+        }
+        // entries of weight 0 in a full row count too (jet:418-428)
+        u32 r = 0;
+        if (active && row_full(d, u, r)) {
+            for (u32 i = lane; i < d.W; i += G) {
+                u32 z = d.wV[(u64) r * d.W + i] & ~d.wP[(u64) r * d.W + i];
+                if (own_b >> 5 == i) z &= ~(1u << (own_b & 31));
+                while (z) {
+                    const u32 b = i * 32 + (u32) __builtin_ctz(z);
+                    z &= z - 1;
+                    count += d.bw[b] < mbw;
+                }
             }
         }
         sum = group_sum<G>(sum, red2);
@@ -1502,6 +1563,7 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
                 delta += mv ? x : 2 * x;
             }
             stale_of(d)[v[q]] = 3;
+            row_touch(d, v[q]);
         }
     }
     return delta;
@@ -1654,6 +1716,7 @@ __device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot, const Je
     rep[ST_MOVES] += tot[PT_COUNT + 1];
     const u32 mode = ctl->mode;
     const u32 n_moves = (u32) rep[ST_MOVES];
+    ctl->last_moves = n_moves;
     account(d, ctl, rep, mode, ctl->save != 0);
     if (d.D) {
         map_control(d, ctl, rep, maxw);
@@ -1901,6 +1964,7 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
                     acc += mv ? x : 2 * x;
                 }
                 stale_of(d)[v[q]] = 3;
+                row_touch(d, v[q]);
             }
         }
         if (t < cnt && !hub) {
@@ -1966,6 +2030,130 @@ __global__ __launch_bounds__(JB) void control_kernel(JetDev d) {
         }
         control_step(d, m, t, in);
     }
+}
+
+// ---------------------------------------------------------------- BlockConn rows of wide vertices
+// update_small / update_large (block_conn.h:208-466) reduced to what a
+// decision reads.  For a wide vertex with a neighbour in the move list, P =
+// the blocks of its neighbours after the moves (every edge weight is
+// positive, so these are its positive entries), and
+//   update_large (a move list longer than n / 10, jet:853): the row is
+//     rebuilt; needed = |P|, size = needed + max(3, needed / 4) if that is
+//     below k, else k; valid entries = P;
+//   update_small: an entry whose weight falls to 0 becomes a tombstone unless
+//     the size is k; a new block takes a tombstone or an empty slot inside the
+//     size before the size grows, so size = max(size, |P|); valid entries = P,
+//     or, in a row of size k, the old valid entries plus P.
+// init_BlockConn (:48-196): size = |P|, valid entries = P.
+enum RowMode : u32 { ROW_SMALL = 0, ROW_LARGE = 1, ROW_INIT = 2 };
+
+template<int G>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (G <= WAVE) wave_sync();
+    else __syncthreads();
+}
+
+// one wide vertex u per group of G lanes; wm: the group's W LDS words
+template<int G>
+__device__ __forceinline__ void row_update(const JetDev &d, u32 *wm, u32 u, u32 lane, u32 rmode, bool active,
+                                           u32 *red) {
+    if (active)
+        for (u32 i = lane; i < d.W; i += G) wm[i] = 0;
+    group_sync<G>();
+    if (active) {
+        const u64 e = d.xadj[u + 1];
+        for (u64 j0 = (u64) d.xadj[u] + lane; j0 < e; j0 += (u64) G * ADJ_UNROLL) {
+            u32 v[ADJ_UNROLL];
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q) v[q] = j0 + (u64) q * G < e ? d.adj[j0 + (u64) q * G] : 0u;
+#pragma unroll
+            for (u32 q = 0; q < ADJ_UNROLL; ++q)
+                if (j0 + (u64) q * G < e) {
+                    const u32 b = d.work[v[q]];
+                    atomicOr(&wm[b >> 5], 1u << (b & 31));
+                }
+        }
+    }
+    group_sync<G>();
+    u32 pc = 0;
+    if (active)
+        for (u32 i = lane; i < d.W; i += G) pc += (u32) __popc(wm[i]);
+    pc = group_sum<G>(pc, red);
+    if (active) {
+        const u32 r = d.wid[u];
+        const u32 sz = d.wsz[r];
+        const bool keep = rmode == ROW_SMALL && sz == d.k;
+        for (u32 i = lane; i < d.W; i += G) {
+            const u64 o = (u64) r * d.W + i;
+            const u32 P = wm[i];
+            d.wV[o] = keep ? (d.wV[o] | P) : P;
+            d.wP[o] = P;
+        }
+        if (lane == 0) {
+            u32 ns;
+            if (rmode == ROW_INIT) {
+                ns = pc;
+            } else if (rmode == ROW_LARGE) {
+                ns = pc + (pc / 4 < 3 ? 3 : pc / 4);
+                if (ns >= d.k) ns = d.k;
+            } else {
+                ns = sz > pc ? sz : pc;
+            }
+            d.wsz[r] = ns;
+            d.rflag[u] = 1;
+        }
+    }
+    group_sync<G>();
+}
+
+// the flagged wide vertices that are not hubs, G lanes each; init: every
+// wide vertex (flagged by the set-up), before the loop starts
+template<int G>
+__global__ __launch_bounds__(JB) void row_kernel(JetDev d, u32 init) {
+    extern __shared__ __align__(16) u32 rsm[];
+    __shared__ u32 s_nlist;
+    u32 rmode = ROW_INIT;
+    if (!init) {
+        const CtlHead hd = ctl_head(d.ctl);
+        if (!live(hd)) return;
+        rmode = d.ctl->last_moves > d.n / 10 ? ROW_LARGE : ROW_SMALL;
+    }
+    const u32 groups = JB / G, lane = threadIdx.x % G, gi = threadIdx.x / G;
+    u32 *list = rsm;
+    u32 *wm = rsm + SWEEP_PIECE + gi * d.W;
+    for (u64 p0 = (u64) blockIdx.x * SWEEP_PIECE; p0 < d.n; p0 += (u64) gridDim.x * SWEEP_PIECE) {
+        const u64 p1 = p0 + SWEEP_PIECE < d.n ? p0 + SWEEP_PIECE : d.n;
+        if (threadIdx.x == 0) s_nlist = 0;
+        __syncthreads();
+        for (u64 v0 = p0; v0 < p1; v0 += JB) {
+            const u64 vv = v0 + threadIdx.x;
+            bool take = false;
+            if (vv < p1 && d.rflag[vv] == 3) take = degree(d, (u32) vv) <= d.hub_deg;
+            append_local(take, (u32) vv, list, &s_nlist);
+        }
+        __syncthreads();
+        const u32 nl = s_nlist;
+        for (u32 r0 = 0; r0 < nl; r0 += groups) {
+            const u32 idx = r0 + gi;
+            row_update<G>(d, wm, idx < nl ? list[idx] : 0u, lane, rmode, idx < nl, nullptr);
+        }
+        __syncthreads();
+    }
+}
+
+// a flagged wide hub: one workgroup
+__global__ __launch_bounds__(JB) void row_hub_kernel(JetDev d, const u32 *hubs, u32 init) {
+    extern __shared__ __align__(16) u32 rsm[];
+    __shared__ u32 red[JB / WAVE];
+    u32 rmode = ROW_INIT;
+    if (!init) {
+        const CtlHead hd = ctl_head(d.ctl);
+        if (!live(hd)) return;
+        rmode = d.ctl->last_moves > d.n / 10 ? ROW_LARGE : ROW_SMALL;
+    }
+    const u32 u = hubs[blockIdx.x];
+    if (d.rflag[u] != 3) return;
+    row_update<JB>(d, rsm, u, threadIdx.x, rmode, true, red);
 }
 
 // ---------------------------------------------------------------- single-workgroup form
@@ -2261,6 +2449,32 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             control_step(d, m, t, *d.ctl);
         }
         __syncthreads();
+        // ---- rows of the wide vertices next to the moves (the group's
+        // connectivity counters, all zero between sweeps, hold its mask words)
+        if (d.rflag && nm) {
+            const u32 rmode = nm > d.n / 10 ? ROW_LARGE : ROW_SMALL;
+            u32 *wm = (u32 *) gc.cnt;
+            for (u32 p0 = 0; p0 < d.n; p0 += BM_PIECE) {
+                const u32 p1 = p0 + BM_PIECE < d.n ? p0 + BM_PIECE : d.n;
+                if (tid == 0) s_nlist = 0;
+                __syncthreads();
+                for (u32 v0 = p0; v0 < p1; v0 += BM) {
+                    const u32 v = v0 + tid;
+                    append_local(v < p1 && d.rflag[v] == 3, v, list, &s_nlist);
+                }
+                __syncthreads();
+                const u32 nl = s_nlist;
+                for (u32 r0 = 0; r0 < nl; r0 += groups) {
+                    const u32 idx = r0 + gi;
+                    const bool act = ingroup && idx < nl;
+                    row_update<G>(d, wm, act ? list[idx] : 0u, lane, rmode, act, nullptr);
+                }
+                __syncthreads();
+            }
+            if (ingroup)
+                for (u32 i = lane; i < d.W; i += G) wm[i] = 0;
+            __syncthreads();
+        }
     }
     if constexpr (LS) {
         __syncthreads();
@@ -2414,6 +2628,24 @@ struct OverDst {
     }
 };
 
+// wide vertices (row capacity min(deg, k) = k): ranks and flags
+struct WideSrc {
+    const u32 *xadj;
+    u32 k;
+    __device__ u32 operator()(u64 u) const { return xadj[u + 1] - xadj[u] >= k ? 1u : 0u; }
+};
+struct WideDst {
+    const u32 *xadj;
+    u32 k;
+    u32 *wid;
+    u8 *rflag;
+    __device__ void operator()(u64 u, u32 pos) const {
+        const bool w = xadj[u + 1] - xadj[u] >= k;
+        wid[u] = w ? pos : ~0u;
+        rflag[u] = w ? 3 : 0;  // the init pass builds every wide row
+    }
+};
+
 // ---------------------------------------------------------------- host driver
 struct JetHost {
     Ctx &c;
@@ -2433,8 +2665,25 @@ struct JetHost {
     bool flat = false;  // edge-balanced afterburner and apply (skewed degree distribution)
     u32 n_chunks = 0;  // hub chunks of the apply step
     u32 *chunk_hub = nullptr, *chunk_idx = nullptr;
+    bool rows = false;  // the level has wide vertices (degree >= k)
 
     JetHost(Ctx &cc, const DGraph &gg, const GraphMeta &mm) : c(cc), g(gg), meta(mm) {}
+
+    template<int GG>
+    void launch_rows(u32 init) {
+        const size_t smem = 4ull * SWEEP_PIECE + 4ull * (JB / GG) * d.W;
+        launch(c, "jet_rows", 0.0, dim3(grid_for(c, div_up(d.n, SWEEP_PIECE), 1, 1)), dim3(JB), smem,
+               row_kernel<GG>, d, init);
+    }
+    // the rows of the wide vertices next to the last iteration's moves (or, at
+    // init, of every wide vertex)
+    void row_update_launches(u32 init) {
+        if (!rows) return;
+        HEIPA_DISPATCH_G(G, launch_rows, init);
+        if (meta.n_hubs)
+            launch(c, "jet_rows", 0.0, dim3(meta.n_hubs), dim3(JB), 4ull * d.W, row_hub_kernel, d,
+                   (const u32 *) meta.hubs, init);
+    }
 
     template<int GG>
     void launch_sweep() {
@@ -2448,6 +2697,7 @@ struct JetHost {
         if (flat) launch(c, "jet_apply", 0.0, dim3(d.nreg), dim3(JB), 0, apply_kernel, d);
         else HEIPA_DISPATCH_G(G, launch_apply);
         launch(c, "jet_control", 0.0, dim3(1), dim3(JB), 0, control_kernel, d);
+        row_update_launches(0);
     }
 
     template<int GG>
@@ -2681,6 +2931,20 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         launch(c, "jet_init", 0.0, hg, dim3(256), 0, hub_chunk_fill_kernel, (const u32 *) cnt, (const u32 *) off,
                meta.n_hubs, h.chunk_hub, h.chunk_idx);
     }
+    // BlockConn rows of the wide vertices (block_conn.h)
+    if (meta.max_deg >= k) {
+        h.rows = true;
+        d.W = (k + 31) / 32;
+        d.rflag = bs.get<u8>(n);
+        d.wid = bs.get<u32>(n);
+        u32 *cnt = bs.get<u32>(1);
+        scan<u32>(c, "jet_init", WideSrc{g.xadj, k}, WideDst{g.xadj, k, d.wid, d.rflag}, n, cnt);
+        u32 nw = 0;
+        readback(c, &nw, cnt, 1);
+        d.wsz = bs.get<u32>(nw ? nw : 1);
+        d.wV = bs.get<u32>((u64) (nw ? nw : 1) * d.W);
+        d.wP = bs.get<u32>((u64) (nw ? nw : 1) * d.W);
+    }
     copy_d2d(c, d.work, part, 4ull * n);
     fill<u32>(c, d.dcache, n, NULL_PART);
     fill<s64>(c, d.icache, n, -1);
@@ -2690,6 +2954,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     fill<s64>(c, d.btot, (u64) d.nshard * ((u64) d.t + 1), 0);
     block_weights_dev(c, g, d.work, k, d.bw);
     launch(c, "jet_init", 0.0, dim3(1), dim3(64), 0, ctl_init_kernel, d, rt, cut, maxw);
+    h.row_update_launches(1);  // init_BlockConn (jet:951)
 
     // Batches of iterations; the host reads the control block (and the trace
     // ring) of batch i while batch i + 1 is already queued.  The launches of a

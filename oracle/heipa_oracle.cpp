@@ -427,6 +427,226 @@ struct Conn {
     }
 };
 
+// BlockConn (block_conn.h) restated slot by slot, in serial order.  Only two
+// decisions read it beyond the positive connections that Conn recomputes from
+// the adjacency: the strong-rebalance mean counts every valid entry of a
+// light block (jet:418-428, promap:387-399), and the mapping LP takes every
+// valid entry as a candidate destination (promap:237-246).  A valid entry can
+// hold weight 0: update_small keeps an entry whose weight drops to 0 while the
+// row's size is k (block_conn.h:379), until update_large rebuilds the row.
+// Slot positions follow serial execution; no decision depends on them (ties
+// go to the smallest block id, rule D4).
+constexpr u32 ROW_EMPTY = 0xFFFFFFFFu;    // NULL_PART in the rows
+constexpr u32 ROW_RECLAIM = 0xFFFFFFFEu;  // HASH_RECLAIM
+int g_row_mode = 1;   // 1: the reference's rows; 0: positive connections only (rule D4 as first written)
+int g_row_check = 0;  // 1: after every update, every row's positive entries must equal the adjacency's
+int g_row_error = 0;  // set when a check or a capacity bound fails
+
+struct RowTable {
+    const G &g;
+    u32 k;
+    std::vector<u32> row, sizes, ids;
+    std::vector<s64> wts;
+    RowTable(const G &gg, u32 kk) : g(gg), k(kk) {}
+    u32 cap(u32 u) const { return row[u + 1] - row[u]; }
+
+    // init_BlockConn (block_conn.h:48-196): capacity min(deg, k); each edge
+    // finds its block's slot or claims the first empty one from slot 0
+    void init(const std::vector<u32> &part) {
+        row.assign(g.n + 1, 0);
+        for (u32 u = 0; u < g.n; ++u) row[u + 1] = row[u] + std::min(g.deg(u), k);
+        sizes.assign(g.n, 0);
+        ids.assign(row[g.n], ROW_EMPTY);
+        wts.assign(row[g.n], 0);
+        for (u32 u = 0; u < g.n; ++u)
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 b = part[g.adj[j]];
+                for (u32 s = row[u]; s < row[u + 1]; ++s) {
+                    if (ids[s] == b) {
+                        wts[s] += g.w(j);
+                        break;
+                    }
+                    if (ids[s] == ROW_EMPTY) {
+                        ids[s] = b;
+                        wts[s] += g.w(j);
+                        sizes[u]++;
+                        break;
+                    }
+                }
+            }
+    }
+
+    // update_small (block_conn.h:347-466): every move's weight leaves its old
+    // block's entry in each neighbour's row (an entry that drops to 0 becomes a
+    // tombstone unless the row's size is k), then enters the new block's entry
+    // (found, or a tombstone / empty slot on its probe path, or the slot after
+    // the row's size, which grows)
+    void update_small(const std::vector<u32> &moves, const std::vector<u32> &old_part,
+                      const std::vector<u32> &part) {
+        for (u32 u: moves) {
+            const u32 ob = old_part[u];
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j], r = row[v], len = sizes[v];
+                u32 idx = ob % len, steps = 0;
+                while (ids[r + idx] != ob) {
+                    idx = (idx + 1) % len;
+                    if (++steps > len) { g_row_error = 1; return; }
+                }
+                const s64 before = wts[r + idx];
+                wts[r + idx] -= g.w(j);
+                if (len != k && before == g.w(j)) ids[r + idx] = ROW_RECLAIM;
+            }
+        }
+        for (u32 u: moves) {
+            const u32 nb = part[u];
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 v = g.adj[j], r = row[v], len = sizes[v];
+                u32 idx = nb % len;
+                bool ok = false;
+                for (u32 t = 0; t < len; ++t) {
+                    idx = (nb + t) % len;
+                    if (ids[r + idx] == nb) { ok = true; break; }
+                    if (ids[r + idx] == ROW_EMPTY) break;
+                }
+                if (!ok)
+                    for (u32 t = 0; t < len; ++t) {
+                        idx = (nb + t) % len;
+                        const u32 id = ids[r + idx];
+                        if (id == nb) { ok = true; break; }
+                        if (id == ROW_EMPTY || id == ROW_RECLAIM) {
+                            ids[r + idx] = nb;
+                            ok = true;
+                            break;
+                        }
+                    }
+                if (!ok) {
+                    idx = len;
+                    while (true) {
+                        if (idx >= cap(v)) { g_row_error = 1; return; }
+                        const u32 id = ids[r + idx];
+                        if (id == nb) break;
+                        if (id == ROW_EMPTY || id == ROW_RECLAIM) {
+                            ids[r + idx] = nb;
+                            sizes[v]++;
+                            break;
+                        }
+                        idx++;
+                    }
+                }
+                wts[r + idx] += g.w(j);
+            }
+        }
+    }
+
+    // update_large (block_conn.h:208-345): the row of every vertex with a
+    // moved neighbour is rebuilt from its adjacency (block b at slot b when the
+    // capacity is k, else linear probing from b mod capacity), then shrunk to
+    // needed + max(3, needed / 4) slots if that is smaller, rehashing in slot
+    // order
+    void update_large(const std::vector<u32> &moves, const std::vector<u32> &part) {
+        std::vector<u8> moved(g.n, 0);
+        for (u32 u: moves) moved[u] = 1;
+        std::vector<u32> sid;
+        std::vector<s64> sw;
+        for (u32 u = 0; u < g.n; ++u) {
+            bool need = false;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1] && !need; ++j) need = moved[g.adj[j]];
+            if (!need) continue;
+            const u32 r = row[u], len = cap(u);
+            sid.assign(len, ROW_EMPTY);
+            sw.assign(len, 0);
+            u32 needed = 0;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 b = part[g.adj[j]];
+                u32 idx = b % len;
+                if (len == k) {
+                    if (sid[idx] == ROW_EMPTY) {
+                        sid[idx] = b;
+                        needed++;
+                    }
+                } else {
+                    while (true) {
+                        if (sid[idx] == ROW_EMPTY) {
+                            sid[idx] = b;
+                            needed++;
+                            break;
+                        }
+                        if (sid[idx] == b) break;
+                        idx = (idx + 1) % len;
+                    }
+                }
+                sw[idx] += g.w(j);
+            }
+            for (u32 s = 0; s < len; ++s) {
+                ids[r + s] = ROW_EMPTY;
+                wts[r + s] = 0;
+            }
+            const u32 new_size = needed + std::max<u32>(3, needed / 4);
+            if (new_size < len) {
+                sizes[u] = new_size;
+                for (u32 s = 0; s < len; ++s) {
+                    if (sid[s] == ROW_EMPTY) continue;
+                    u32 idx = sid[s] % new_size;
+                    while (ids[r + idx] != ROW_EMPTY && ids[r + idx] != sid[s]) idx = (idx + 1) % new_size;
+                    ids[r + idx] = sid[s];
+                    wts[r + idx] = sw[s];
+                }
+            } else {
+                sizes[u] = len;
+                for (u32 s = 0; s < len; ++s) {
+                    ids[r + s] = sid[s];
+                    wts[r + s] = sw[s];
+                }
+            }
+        }
+    }
+
+    // perform_moves' choice (jet:853-857, promap:1161-1165)
+    void update(const std::vector<u32> &moves, const std::vector<u32> &old_part, const std::vector<u32> &part) {
+        if (moves.size() > g.n / 10) update_large(moves, part);
+        else update_small(moves, old_part, part);
+        if (g_row_check) check(part);
+    }
+
+    // every row's entries with positive weight are exactly the vertex's
+    // positive block connections; no valid entry lies beyond the row's size
+    void check(const std::vector<u32> &part) {
+        std::vector<s64> acc(k, 0);
+        for (u32 u = 0; u < g.n; ++u) {
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) acc[part[g.adj[j]]] += g.w(j);
+            u32 pos = 0;
+            for (u32 s = row[u]; s < row[u + 1]; ++s) {
+                const u32 id = ids[s];
+                if (id == ROW_EMPTY || id == ROW_RECLAIM) continue;
+                if (s >= row[u] + sizes[u] || id >= k || wts[s] != acc[id] || wts[s] < 0) g_row_error = 1;
+                pos += wts[s] > 0;
+            }
+            u32 distinct = 0;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) {
+                const u32 b = part[g.adj[j]];
+                if (acc[b] > 0) {
+                    distinct++;
+                    acc[b] = -acc[b];  // count once
+                }
+            }
+            if (distinct != pos) g_row_error = 1;
+            for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) acc[part[g.adj[j]]] = 0;
+        }
+    }
+
+    // calls f(b) for every valid entry of u's row other than skip (the
+    // reference walks [row, row + size) in strong rebalance and the whole
+    // capacity in the mapping LP; entries beyond the size are empty)
+    template<typename F>
+    void for_valid(u32 u, u32 skip, F &&f) const {
+        for (u32 s = row[u]; s < row[u] + sizes[u]; ++s) {
+            const u32 id = ids[s];
+            if (id == ROW_EMPTY || id == ROW_RECLAIM || id == skip) continue;
+            f(id, wts[s]);
+        }
+    }
+};
+
 struct Jet {
     const G &g;
     u32 k;
@@ -438,8 +658,15 @@ struct Jet {
     std::vector<u8> lock;
     Conn conn;
     s64 cut = 0, maxw = 0;
+    RowTable rows;
+    bool use_rows = false;
 
-    Jet(const G &gg, u32 kk, s64 lm) : g(gg), k(kk), lmax(lm), conn(kk) {}
+    Jet(const G &gg, u32 kk, s64 lm) : g(gg), k(kk), lmax(lm), conn(kk), rows(gg, kk) {}
+
+    void init_rows() {  // init_BlockConn at the start of the refine call (jet:951, promap:1244)
+        use_rows = g_row_mode == 1;
+        if (use_rows) rows.init(part);
+    }
 
     void compute_sections(u32 &sections, u32 &t, u32 &width) const {  // jet:364-371, :605-612
         u32 s = MAX_SECTIONS;
@@ -617,11 +844,20 @@ struct Jet {
             conn.compute(g, part, u);
             const s64 own = conn.acc[u_id];
             s64 count = 0, sum = 0;
-            for (u32 b: conn.touched) {
-                if (b == u_id) continue;
-                if (bw[b] < max_b_w) {
-                    sum += conn.acc[b];
-                    count += 1;
+            if (use_rows) {  // every valid entry of a light block counts, weight 0 included
+                rows.for_valid(u, u_id, [&](u32 b, s64 c) {
+                    if (bw[b] < max_b_w) {
+                        sum += c;
+                        count += 1;
+                    }
+                });
+            } else {
+                for (u32 b: conn.touched) {
+                    if (b == u_id) continue;
+                    if (bw[b] < max_b_w) {
+                        sum += conn.acc[b];
+                        count += 1;
+                    }
                 }
             }
             conn.reset();
@@ -709,14 +945,17 @@ struct Jet {
             const s64 own_cost = conn_cost(u_id);
             u32 best_id = NO_MOVE;
             s64 best_delta = -1;  // :232: a move needs delta >= 0
-            for (u32 b: conn.touched) {  // D4
-                if (b == u_id) continue;
+            auto consider = [&](u32 b) {  // D4
                 const s64 delta = own_cost - conn_cost(b);
                 if (delta > best_delta || (delta == best_delta && best_id != NO_MOVE && b < best_id)) {
                     best_delta = delta;
                     best_id = b;
                 }
-            }
+            };
+            if (use_rows) rows.for_valid(u, u_id, [&](u32 b, s64) { consider(b); });  // zero entries too
+            else
+                for (u32 b: conn.touched)
+                    if (b != u_id) consider(b);
             conn.reset();
             gain_cache[u] = best_id != NO_MOVE ? best_delta : 0;
             dest_cache[u] = best_id;
@@ -781,9 +1020,12 @@ struct Jet {
             dest_cache[u] = NULL_PART;  // :1133, block_conn.h update_small/update_large
             for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) dest_cache[g.adj[j]] = NULL_PART;
         }
+        std::vector<u32> old_part;
+        if (use_rows) old_part = part;
         for (u32 u: moves) part[u] = dest_part[u];
         maxw = 0;
         for (u32 b = 0; b < k; ++b) maxw = std::max(maxw, bw[b]);
+        if (use_rows) rows.update(moves, old_part, part);  // promap:1159-1166
     }
 
     // perform_moves (jet:753-924) with the exact 64-bit cut change
@@ -808,9 +1050,12 @@ struct Jet {
             dest_cache[u] = NULL_PART;
             for (u32 j = g.xadj[u]; j < g.xadj[u + 1]; ++j) dest_cache[g.adj[j]] = NULL_PART;
         }
+        std::vector<u32> old_part;
+        if (use_rows) old_part = part;
         for (u32 u: moves) part[u] = dest_part[u];
         maxw = 0;
         for (u32 b = 0; b < k; ++b) maxw = std::max(maxw, bw[b]);
+        if (use_rows) rows.update(moves, old_part, part);  // jet:851-857
     }
 };
 
@@ -834,6 +1079,7 @@ void jet_refine(const G &g, std::vector<u32> &part, u32 k, s64 lmax, bool ultra,
     J.lock.assign(g.n, 0);
     J.cut = cut;
     J.maxw = maxw;
+    J.init_rows();
     s64 best_cut = cut, best_maxw = maxw;
     std::vector<double> ratios;
     if (ultra) ratios = {0.85, 0.80, 0.75, 0.70, 0.65, 0.60, 0.55, 0.50, 0.45, 0.40, 0.35, 0.30, 0.25, 0.20, 0.15, 0.10, 0.05};
@@ -919,6 +1165,7 @@ void map_refine(const G &g, std::vector<u32> &part, const s32 *D, u32 k, s64 lma
     X.lock.assign(g.n, 0);
     X.cut = J;
     X.maxw = maxw;
+    X.init_rows();
     s64 best_J = J, best_maxw = maxw;
     u32 tl = 0, bal = 0, it = 0;
     while (it < N_MAX_ITERATIONS) {
@@ -1240,6 +1487,12 @@ bool im_solve(const G &g0, const std::vector<u32> &hier, const std::vector<s32> 
 
 extern "C" {
 
+void orc_set_row_mode(int mode) { g_row_mode = mode; }
+void orc_set_row_check(int on) {
+    g_row_check = on;
+    g_row_error = 0;
+}
+int orc_row_error(void) { return g_row_error; }
 uint32_t orc_murmur3_x86_32(const void *key, int32_t len, uint32_t seed) { return murmur3(key, len, seed); }
 uint32_t orc_pod_hash_u32(uint32_t x) { return pod_hash(x); }
 uint32_t orc_xorshift(uint32_t x) { return xorshift(x); }

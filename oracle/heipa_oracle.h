@@ -65,6 +65,14 @@ typedef struct {
 } orc_match_stats;
 
 /* hashing (two_hop_matching.h:42, :93-100; Kokkos pod_hash = MurmurHash3_x86_32) */
+/* Block connectivity rows (block_conn.h).  mode 1 (default): the reference's
+ * rows, whose entries can keep weight 0 (they count in strong rebalance and are
+ * mapping-LP candidates); mode 0: positive connections only.  With the check
+ * on, every row is compared with the adjacency after each update; a mismatch
+ * or a capacity overrun sets orc_row_error(). */
+void orc_set_row_mode(int mode);
+void orc_set_row_check(int on);
+int orc_row_error(void);
 uint32_t orc_murmur3_x86_32(const void *key, int32_t len, uint32_t seed);
 uint32_t orc_pod_hash_u32(uint32_t x);
 uint32_t orc_xorshift(uint32_t x);

@@ -58,6 +58,9 @@ def lib():
         G = ctypes.POINTER(OrcGraph)
         c_u32, c_u64, c_int, c_dbl = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
         sig = {
+            "orc_set_row_mode": (None, [c_int]),
+            "orc_set_row_check": (None, [c_int]),
+            "orc_row_error": (c_int, []),
             "orc_murmur3_x86_32": (c_u32, [ctypes.c_void_p, ctypes.c_int32, c_u32]),
             "orc_pod_hash_u32": (c_u32, [c_u32]),
             "orc_xorshift": (c_u32, [c_u32]),
@@ -114,6 +117,21 @@ class _G:
     @property
     def ref(self):
         return ctypes.byref(self.s)
+
+
+def set_row_mode(mode):
+    """1 (default): block connectivity rows as the reference keeps them
+    (block_conn.h), entries of weight 0 included; 0: positive connections only."""
+    lib().orc_set_row_mode(int(mode))
+
+
+def set_row_check(on):
+    """Compare every row with the adjacency after each update (slow)."""
+    lib().orc_set_row_check(int(on))
+
+
+def row_error():
+    return lib().orc_row_error()
 
 
 def murmur3(data: bytes, seed: int) -> int:

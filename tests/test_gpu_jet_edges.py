@@ -11,7 +11,7 @@ import pytest
 
 import heipa
 import heipa_oracle as orc
-from brute import heavy_block_case, slight_overload_case, vertex_weighted
+from brute import heavy_block_case, slight_overload_case, two_heavy_blocks_case, vertex_weighted
 from graphs import block_weights, random_partition
 from test_gpu_parity import _dev, _trace_tuple
 
@@ -53,6 +53,11 @@ def _cases():
     out["grid8k6"] = _random_case(lambda: heipa.gen_grid(8, 8), 6, 1.0, 2)
     out["rgg70vw"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(70, 6.0, 4, 9), 4, 40), 3, 3.0, 2)
     out["rgg50vwk6"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(50, 5.0, 7, 9), 5, 60), 6, 0.0, 3)
+    # strong candidates of block 0 in bucket 49 leak into block 1's range (A11)
+    for s in (1, 2):
+        out[f"leak{s}"] = two_heavy_blocks_case(s)[:4]
+    # full rows keep entries of weight 0 that strong rebalance counts
+    out["zerorow"] = _random_case(lambda: vertex_weighted(heipa.gen_rgg(150, 7.0, 4, 9), 4, 80), 4, 1.0, 3)
     # lmax within 0.1-1% of W / k: strong evicts by min(weight, excess)
     for k, eps, skew, seed in ((4, 0.001, 0.5, 1), (2, 0.003, 2.0, 1), (3, 0.01, 2.0, 2)):
         g = vertex_weighted(heipa.gen_grid(8, 8), 3, 20)
@@ -119,6 +124,30 @@ def test_map_refine_edges(ctx, name, form):
         _same(name, gtr, otr)
         assert (gJ, gmaxw) == (oJ, omaxw)
         assert np.array_equal(heipa.to_numpy_u32(pd), op), name
+
+    _with_form(ctx, form, run)
+
+
+@pytest.mark.parametrize("form", sorted(FORMS))
+def test_map_refine_zero_entries(ctx, form):
+    """Distances 7:1 (not growing with the level): a block whose connection
+    fell to 0 in a full row wins the mapping LP (promap:237-246)."""
+    g = heipa.gen_rmat(7, 0, 8.0, 0.57, 0.19, 0.19, 6, 9, True)
+    D = orc.distance_matrix([3, 2], [7, 1])
+    k = D.shape[0]
+    part0 = random_partition(g.n, k, 4, 3.0)
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    Dd = heipa.from_numpy_u32(D.reshape(-1).view(np.uint32))
+    J0, maxw0 = orc.comm_cost(g, part0, D), int(block_weights(g, part0, k).max())
+    op, oJ, omaxw, otr = orc.map_refine(g, part0, D, lmax, J0, maxw0, trace_cap=4096)
+    print(f"map zero entries form={form}", flush=True)
+
+    def run():
+        pd = heipa.from_numpy_u32(part0)
+        gJ, gmaxw, gtr = ctx.map_refine(_dev(g), pd, Dd, k, lmax, 0, J0, maxw0, trace_cap=4096)
+        _same("zero", gtr, otr)
+        assert (gJ, gmaxw) == (oJ, omaxw)
+        assert np.array_equal(heipa.to_numpy_u32(pd), op)
 
     _with_form(ctx, form, run)
 

@@ -21,7 +21,7 @@ import pytest
 
 import heipa
 import heipa_oracle as orc
-from brute import heavy_block_case, strong_iteration, vertex_weighted, weak_iteration
+from brute import Rows, heavy_block_case, strong_iteration, vertex_weighted, weak_iteration
 from graphs import block_weights, random_partition
 
 GAIN_MIN = -(1 << 63)
@@ -96,17 +96,22 @@ def test_map_lp_first_iteration_brute_force(name, make, hier, dist, seed):
     assert (t.n_cand, t.n_moves, t.gain_hash, t.moves_hash, t.cut) == (n_cand, len(moves), gh, mh, J1)
 
 
-def _map_lp_iteration(g, part, D, cache, locked):
+def _map_lp_iteration(g, part, D, cache, locked, rows=None):
     """One ProMap LP iteration (promap:206-330) with the decision cache and
     locks of the reference (a decision is recomputed only after the vertex or
     a neighbour moved; a vertex moved by the previous LP iteration is no
-    candidate), decisions and afterburner from whole-graph J evaluations."""
+    candidate), decisions and afterburner from whole-graph J evaluations.
+    With `rows`, a row entry of weight 0 is a candidate destination too
+    (promap:237-246 walks every valid entry)."""
     J0 = _J(g, part, D)
     for u in range(g.n):
         if u in cache:
             continue
         best, best_b = -1, None
-        for b in sorted({int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]} - {int(part[u])}):
+        cands = {int(part[v]) for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]}
+        if rows:
+            cands |= rows.zero_entries(part, u)
+        for b in sorted(cands - {int(part[u])}):
             d2 = J0 - _J(g, _moved(part, {u: b}), D)
             if d2 // 2 > best:
                 best, best_b = d2 // 2, b
@@ -126,7 +131,7 @@ def _map_lp_iteration(g, part, D, cache, locked):
     return len(gain), moves, dest, gh
 
 
-def _map_refine(g, part, D, lmax):
+def _map_refine(g, part, D, lmax, use_rows=True):
     """ProMap_jet_refine (promap:1219-1311) over brute-force iterations: at
     most 12 iterations; LP while balanced, else the balance counter goes up
     first and its first value runs a weak iteration (the rebalancing is the
@@ -140,19 +145,23 @@ def _map_refine(g, part, D, lmax):
     maxw = int(np.bincount(part, weights=w, minlength=k).max())
     best = (part.copy(), J, maxw)
     cache, locked, trace = {}, set(), []
+    rows = Rows(g, part, k) if use_rows else None
     bal = it = 0
     while it < 12:
         it += 1
         gh = 0
         if maxw <= lmax:
             mode = 0
-            n_cand, moves, dest, gh = _map_lp_iteration(g, part, D, cache, locked)
+            n_cand, moves, dest, gh = _map_lp_iteration(g, part, D, cache, locked, rows)
             locked = set(moves)
             bal = 0
         else:
             bal += 1
             mode = 1 if bal < 2 else 2
-            n_cand, moves, dest = (weak_iteration if mode == 1 else strong_iteration)(g, part, k, lmax)
+            if mode == 1:
+                n_cand, moves, dest = weak_iteration(g, part, k, lmax)
+            else:
+                n_cand, moves, dest = strong_iteration(g, part, k, lmax, rows)
         if not moves:
             continue
         mh = sum(orc.mix64((u << 32) | dest[u]) for u in moves) & U64
@@ -161,6 +170,8 @@ def _map_refine(g, part, D, lmax):
             for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]:
                 cache.pop(int(v), None)
         part = _moved(part, {u: dest[u] for u in moves})
+        if rows:
+            rows.update(part, moves)
         J = _J(g, part, D)
         maxw = int(np.bincount(part, weights=w, minlength=k).max())
         trace.append((it, mode, n_cand, len(moves), J, maxw, mh, gh))
@@ -184,6 +195,37 @@ MAP_LOOP = [
     ("grid8x8-k6", lambda: heipa.gen_grid(8, 8), [3, 2], [1, 5], 1.0, 1),
     ("rgg50vw-k6", lambda: vertex_weighted(heipa.gen_rgg(50, 5.0, 7, 9), 5, 60), [2, 3], [2, 9], 0.0, 3),
 ]
+
+
+def test_map_lp_takes_the_zero_entries_of_full_rows():
+    """The mapping LP walks every valid row entry (promap:237-246), so a block
+    whose connection fell to 0 in a full row (block_conn.h:379) is a
+    candidate destination.  With distances that grow with the hierarchy
+    level D is an ultrametric and a neighbour's block always beats such a
+    block; with distances 7:1 (a sibling block is farther than a block of
+    the other group) it can win, here in an LP iteration.  The brute force
+    with the rows' counts equals the oracle; without them it equals the
+    oracle with positive connections only; the two differ."""
+    g = heipa.gen_rmat(7, 0, 8.0, 0.57, 0.19, 0.19, 6, 9, True)
+    D = orc.distance_matrix([3, 2], [7, 1])
+    k = D.shape[0]
+    part = random_partition(g.n, k, 4, 3.0)
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    J0, maxw0 = orc.comm_cost(g, part, D), int(block_weights(g, part, k).max())
+    results = []
+    try:
+        for mode in (1, 0):
+            orc.set_row_mode(mode)
+            (bp, bJ, bm), trace = _map_refine(g, part, D, lmax, use_rows=mode == 1)
+            p, J, mw, tr = orc.map_refine(g, part, D, lmax, J0, maxw0, trace_cap=4096)
+            got = [(t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash) for t in tr]
+            assert got == trace
+            assert np.array_equal(p, bp) and (J, mw) == (bJ, bm)
+            results.append(trace)
+    finally:
+        orc.set_row_mode(1)
+    first = next(i for i, (a, b) in enumerate(zip(*results)) if a != b)
+    assert results[0][first][1] == 0  # an LP iteration
 
 
 @pytest.mark.parametrize("name,make,hier,dist,skew,seed", MAP_LOOP)

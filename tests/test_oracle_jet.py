@@ -38,7 +38,7 @@ import pytest
 
 import heipa
 import heipa_oracle as orc
-from brute import U64, cut as _cut, heavy_block_case, moved as _moved, slight_overload_case, strong_iteration as _strong_iteration, \
+from brute import U64, Rows, cut as _cut, heavy_block_case, two_heavy_blocks_case, moved as _moved, slight_overload_case, strong_iteration as _strong_iteration, \
     vertex_weighted as _vertex_weighted, weak_iteration as _weak_iteration
 from graphs import block_weights, random_partition
 
@@ -182,9 +182,63 @@ def test_jet_weak_weak_strong_chain_brute_force(name, make, k, skew, seed):
     bw = block_weights(g, part, k)
     _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), int(bw.max()), trace_cap=64)
     assert [t.mode for t in tr[:3]] == [1, 1, 2]
+    rows = Rows(g, part, k)
     for t in tr[:2]:
-        part = _check_iteration(g, t, part, *_weak_iteration(g, part, k, lmax))
-    _check_iteration(g, tr[2], part, *_strong_iteration(g, part, k, lmax))
+        n_cand, moves, dest = _weak_iteration(g, part, k, lmax)
+        part = _check_iteration(g, t, part, n_cand, moves, dest)
+        rows.update(part, moves)
+    _check_iteration(g, tr[2], part, *_strong_iteration(g, part, k, lmax, rows))
+
+
+def test_strong_counts_the_zero_entries_of_full_rows():
+    """A vertex of degree >= k whose row reached all k blocks keeps an entry
+    of weight 0 when its last neighbour in a block leaves (block_conn.h:379);
+    rebalance_strong counts it in the mean connectivity (jet:418-428).  In
+    this chain (two weak iterations, then a strong one) such entries change
+    the strong iteration's moves: the brute force with the rows' counts
+    equals the oracle, the brute force without them equals the oracle with
+    positive connections only, and the two differ."""
+    g = _vertex_weighted(heipa.gen_rgg(150, 7.0, 4, 9), 4, 80)
+    k = 4
+    part = random_partition(g.n, k, 3, 1.0)
+    lmax = math.ceil(1.03 * int(g.g_weight) / k)
+    cut0, maxw0 = orc.edge_cut(g, part), int(block_weights(g, part, k).max())
+    rows = Rows(g, part, k)
+    for _ in range(2):
+        n_cand, moves, dest = _weak_iteration(g, part, k, lmax)
+        part = _moved(part, {u: dest[u] for u in moves})
+        rows.update(part, moves)
+    assert any(rows.zero_entries(part, u) for u in range(g.n))
+    with_rows = _strong_iteration(g, part, k, lmax, rows)
+    without = _strong_iteration(g, part, k, lmax)
+    assert with_rows[1] != without[1] or with_rows[2] != without[2]
+    p0 = random_partition(g.n, k, 3, 1.0)
+    try:
+        for mode, expect in ((1, with_rows), (0, without)):
+            orc.set_row_mode(mode)
+            _, _, _, tr = orc.jet_refine(g, p0, k, lmax, False, 0, cut0, maxw0, trace_cap=64)
+            assert [t.mode for t in tr[:3]] == [1, 1, 2]
+            _check_iteration(g, tr[2], part, *expect)
+    finally:
+        orc.set_row_mode(1)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 5])
+def test_jet_strong_minibucket_leak(seed):
+    """Appendix A11 (jet:437 against :494-495, :527-530): strong candidates in
+    bucket 49 and the last section sit on the first minibucket of the next
+    block's range.  two_heavy_blocks_case puts block 0's cluster there; the
+    brute force that orders them after their own block and counts their
+    weight ahead of every candidate of block 1 equals the oracle, and the
+    brute force without the leak does not."""
+    g, part, k, lmax, bw = two_heavy_blocks_case(seed)
+    _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), max(bw), trace_cap=64)
+    assert tr[0].mode == 2
+    rows = Rows(g, part, k)
+    with_leak = _strong_iteration(g, part, k, lmax, rows)
+    without = _strong_iteration(g, part, k, lmax, rows, a11=False)
+    assert with_leak[1] != without[1] or with_leak[2] != without[2]
+    _check_iteration(g, tr[0], part, *with_leak)
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
@@ -195,7 +249,7 @@ def test_jet_strong_takes_vertices_too_heavy_for_weak(seed):
     g, part, k, lmax, bw = heavy_block_case(seed)
     _, _, _, tr = orc.jet_refine(g, part, k, lmax, False, 0, orc.edge_cut(g, part), bw[0], trace_cap=64)
     assert tr[0].mode == 2
-    _check_iteration(g, tr[0], part, *_strong_iteration(g, part, k, lmax))
+    _check_iteration(g, tr[0], part, *_strong_iteration(g, part, k, lmax, Rows(g, part, k)))
 
 
 def _lp_iteration(g, part, k, ratio, cache, locked):
@@ -248,6 +302,7 @@ def _jet_refine(g, part, k, lmax, ultra, level):
     maxw = int(np.bincount(part, weights=w, minlength=k).max())
     best = (part.copy(), cut, maxw)
     cache, locked, trace = {}, set(), []
+    rows = Rows(g, part, k)
     for ri, ratio in enumerate(ULTRA if ultra else [0.25 if level == 0 else 0.75]):
         bal = it = 0
         while it < 12:
@@ -269,7 +324,7 @@ def _jet_refine(g, part, k, lmax, ultra, level):
                         continue
                 else:
                     mode = 2
-                    n_cand, moves, dest = _strong_iteration(g, part, k, lmax)
+                    n_cand, moves, dest = _strong_iteration(g, part, k, lmax, rows)
                     if not moves:
                         break
                 bal += 1
@@ -279,6 +334,7 @@ def _jet_refine(g, part, k, lmax, ultra, level):
                 for v in g.adjncy[g.xadj[u]:g.xadj[u + 1]]:
                     cache.pop(int(v), None)
             part = _moved(part, {u: dest[u] for u in moves})
+            rows.update(part, moves)
             cut = _cut(g, part)
             maxw = int(np.bincount(part, weights=w, minlength=k).max())
             trace.append((ri, it, mode, n_cand, len(moves), cut, maxw, mh, gh))
