@@ -111,6 +111,12 @@ int heipa_arena_release(heipa_ctx *ctx, uint64_t mark) {
 
 uint64_t heipa_arena_peak(const heipa_ctx *ctx) { return ctx ? ctx->c.arena.peak() : 0; }
 
+int heipa_arena_reset_peak(heipa_ctx *ctx) {
+    if (!ctx) return HEIPA_ERR_INVALID;
+    ctx->c.arena.reset_peak();
+    return HEIPA_OK;
+}
+
 int heipa_download(heipa_ctx *ctx, void *host_dst, const void *dev_src, uint64_t bytes) {
     return guarded(ctx, [&](Ctx &c) {
         require(bytes == 0 || (host_dst && dev_src), "null argument");
@@ -420,12 +426,50 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         else if (option == HEIPA_OPT_HM_WORKERS) {
             c.hm_workers = (u32) std::min<uint64_t>(value, 64);
             ctx_trim_workers(c, c.hm_workers > 1 ? c.hm_workers : 0);  // frees the arenas of the ones dropped
+        } else if (option == HEIPA_OPT_RACE) {
+            c.race = value != 0;
+        } else if (option == HEIPA_OPT_HIER_CACHE) {
+            if (value) {
+                c.hcache = std::make_shared<HierCache>();
+                c.hcache->budget = (size_t) value;
+            } else {
+                c.hcache.reset();
+            }
+        } else if (option == HEIPA_OPT_GRAPH_KEY) {
+            c.hkey = value;
         } else if (option == HEIPA_OPT_HM_WORKER_BYTES) {
             c.hm_worker_bytes = value;
             ctx_trim_workers(c, 0);  // reopened at the next split with the new size
         }
         else require(false, "unknown option");
     });
+}
+
+int heipa_ctx_share_hier_cache(heipa_ctx *dst, const heipa_ctx *src) {
+    if (!src) return HEIPA_ERR_INVALID;
+    return guarded(dst, [&](Ctx &c) {
+        c.sync();
+        c.hcache = src->c.hcache;
+    });
+}
+
+int heipa_hier_cache_stats(const heipa_ctx *ctx, uint64_t *entries, uint64_t *bytes, uint64_t *levels_reused,
+                           uint64_t *levels_built) {
+    if (!ctx) return HEIPA_ERR_INVALID;
+    const std::shared_ptr<HierCache> hc = ctx->c.hcache;
+    uint64_t e = 0, b = 0, r = 0, bl = 0;
+    if (hc) {
+        std::lock_guard<std::mutex> lk(hc->m);
+        e = hc->entries.size();
+        b = hc->bytes;
+        r = hc->levels_reused;
+        bl = hc->levels_built;
+    }
+    if (entries) *entries = e;
+    if (bytes) *bytes = b;
+    if (levels_reused) *levels_reused = r;
+    if (levels_built) *levels_built = bl;
+    return HEIPA_OK;
 }
 
 int heipa_prof_filter(heipa_ctx *ctx, const char *prefix) {

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -114,6 +115,7 @@ public:
         while (bstack_.size() > depth) pop_back();
     }
     size_t peak() const { return peak_; }
+    void reset_peak() { peak_ = in_use(); }
     size_t capacity() const { return cap_; }
     size_t in_use() const { return front_ + (cap_ - back_); }
 
@@ -142,6 +144,36 @@ struct BackScope {
     T *get(size_t count) { return (T *) a.push_back(sizeof(T) * (count ? count : 1)); }
 };
 
+// Coarse levels of graphs the caller names by a key, kept in device memory
+// between calls.  The hierarchy does not depend on k (matching always uses
+// seed 12345, two_hop_matching.h:590; only the stop level n <= 8k does,
+// solver.h:343-347), so a later solve of the same graph with any k reuses
+// the levels it needs and appends the ones it builds beyond them.
+struct HierLevel {
+    DGraph g;           // level i + 1
+    u32 *map = nullptr; // level i -> level i + 1, fine_n entries
+    u32 fine_n = 0;
+};
+struct HierEntry {
+    std::mutex m;
+    u32 n0 = 0, m0 = 0;  // the finest graph this entry was built from
+    s64 gw0 = 0;
+    std::vector<HierLevel> levels;
+    std::vector<void *> mem;
+    size_t bytes = 0;
+    u64 last_use = 0;
+    ~HierEntry() {
+        for (void *p: mem) (void) hipFree(p);
+    }
+};
+struct HierCache {
+    std::mutex m;
+    std::map<u64, std::shared_ptr<HierEntry>> entries;
+    size_t budget = 0, bytes = 0;
+    u64 clock = 0;
+    u64 hits = 0, levels_reused = 0, levels_built = 0;
+};
+
 struct KernelStat {
     u64 launches = 0;  // all launches
     u64 timed = 0;     // launches bracketed by events (every prof_sample-th)
@@ -160,6 +192,9 @@ struct Ctx {
     bool block_lds = true;   // HEIPA_OPT_BLOCK_LDS
     u32 hm_workers = 8;      // HEIPA_OPT_HM_WORKERS
     u64 hm_worker_bytes = 0; // HEIPA_OPT_HM_WORKER_BYTES (0: the estimate)
+    bool race = false;       // HEIPA_OPT_RACE: atomic order instead of the deterministic rules
+    std::shared_ptr<HierCache> hcache;  // HEIPA_OPT_HIER_CACHE (shared by a pool's workers)
+    u64 hkey = 0;                       // HEIPA_OPT_GRAPH_KEY: the input graph's key (0: no reuse)
     hipStream_t stream = nullptr;
     void *arena_mem = nullptr;
     Arena arena;
@@ -171,6 +206,7 @@ struct Ctx {
     bool prof_on = false;
     std::string prof_filter;  // record only kernels whose name starts with this (empty: all)
     u32 prof_sample = 1;      // bracket about one launch in prof_sample of a kernel with events
+    bool prof_l0 = false;     // launches of a solve's finest level are recorded as "<name>@L0" as well
     std::vector<std::string> prof_names;
     std::map<std::string, size_t> prof_index;
     std::vector<KernelStat> prof_stats;
@@ -195,6 +231,16 @@ struct Ctx {
         return e;
     }
     size_t prof_slot(const char *name) {
+        if (prof_l0) {  // a solve's finest level: its own entry
+            std::string key = std::string(name) + "@L0";
+            auto it = prof_index.find(key);
+            if (it != prof_index.end()) return it->second;
+            size_t i = prof_names.size();
+            prof_names.push_back(key);
+            prof_stats.emplace_back();
+            prof_index.emplace(key, i);
+            return i;
+        }
         auto it = prof_index.find(name);
         if (it != prof_index.end()) return it->second;
         size_t i = prof_names.size();
@@ -296,6 +342,8 @@ inline void ctx_copy_settings(Ctx &to, const Ctx &from) {
     to.block_lds = from.block_lds;
     to.hm_workers = from.hm_workers;
     to.hm_worker_bytes = from.hm_worker_bytes;
+    to.race = from.race;
+    to.hcache = from.hcache;
     to.prof_on = from.prof_on;
     to.prof_filter = from.prof_filter;
     to.prof_sample = from.prof_sample;

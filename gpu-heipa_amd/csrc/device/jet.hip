@@ -187,6 +187,7 @@ struct JetDev {
     u32 *wsz;           // per wide vertex: the row's size (BlockConn sizes)
     u32 *wV, *wP;       // per wide vertex, W words each: blocks of its valid entries / of positive ones
     u32 W;              // words per block mask, ceil(k / 32)
+    u32 race;           // atomic order: minibucket offsets and strong overflow positions from fetch-adds
 };
 
 // Per-vertex staleness of the caches, the n bytes after the locks: bit 0
@@ -201,9 +202,15 @@ __device__ __forceinline__ s64 vwt(const JetDev &d, u32 u) { return d.vw ? (s64)
 // a candidate's weight into minibucket g: rebalancing puts most candidates of
 // an overloaded block into a few dozen minibuckets, so each workgroup adds into
 // its own copy of the totals (the bucket kernel sums the copies)
-__device__ __forceinline__ void btot_add(const JetDev &d, u32 g, s64 w) {
+// Racing mode keeps one copy and records the add's previous total as the
+// vertex's offset in its minibucket (the reference's temp_gain =
+// atomic_fetch_add, jet:439, :684) in gain1, which no rebalancing iteration
+// reads otherwise.
+__device__ __forceinline__ void btot_add(const JetDev &d, u32 u, u32 g, s64 w) {
     const u64 copy = d.nshard > 1 ? blockIdx.x % d.nshard : 0;
-    atomicAdd((unsigned long long *) &d.btot[copy * ((u64) d.t + 1) + g], (unsigned long long) w);
+    const unsigned long long old =
+        atomicAdd((unsigned long long *) &d.btot[copy * ((u64) d.t + 1) + g], (unsigned long long) w);
+    if (d.race) d.gain1[u] = (s64) old;
 }
 __device__ __forceinline__ s64 ewt(const JetDev &d, u32 j) { return d.ew ? (s64) d.ew[j] : 1; }
 __device__ __forceinline__ u32 degree(const JetDev &d, u32 u) { return d.xadj[u + 1] - d.xadj[u]; }
@@ -604,7 +611,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
                 const u32 bt = gain_bucket(gain, vwt(d, u));
                 const u32 g = (MAX_BUCKETS * own_b + bt) * d.sections + (u % d.sections);
                 d.gid[u] = g;
-                btot_add(d, g, vwt(d, u));
+                btot_add(d, u, g, vwt(d, u));
                 listed = true;
             }
         }
@@ -644,7 +651,7 @@ __device__ __forceinline__ bool reb_eval(const JetDev &d, bool strong, s64 mbw, 
             const u32 bt = gain_bucket(gain, wu < lim ? wu : lim);
             const u32 g = (MAX_BUCKETS * own_b + bt) * d.sections + (u % d.sections) + 1;
             d.gid[u] = g;
-            btot_add(d, g, wu);
+            btot_add(d, u, g, wu);
             listed = true;
         }
     }
@@ -665,13 +672,13 @@ __device__ __forceinline__ bool reb_interior(const JetDev &d, bool strong, const
         if (dst == own_b) return false;
         const u32 g = (MAX_BUCKETS * own_b + gain_bucket(-own, wu)) * d.sections + (u % d.sections);
         d.gid[u] = g;
-        btot_add(d, g, wu);
+        btot_add(d, u, g, wu);
         return true;
     }
     const s64 lim = d.bw[own_b] - d.lmax;
     const u32 g = (MAX_BUCKETS * own_b + gain_bucket(-own, wu < lim ? wu : lim)) * d.sections + (u % d.sections) + 1;
     d.gid[u] = g;
-    btot_add(d, g, wu);
+    btot_add(d, u, g, wu);
     return true;
 }
 
@@ -819,6 +826,8 @@ __global__ __launch_bounds__(JB) void bucket_kernel(JetDev d) {
     const CtlHead hd = ctl_head(d.ctl);
     if (!live(hd) || hd.mode == 0) return;
     const u32 p = blockIdx.x;
+    // racing mode: the classify launch records strong evictions' ends here
+    if (d.race && hd.mode == 2 && threadIdx.x == 0) d.evict_adjust[p] = 0;
     const u64 base = (u64) p * d.width;
     const u64 stride = (u64) d.t + 1;
     s64 carry = 0;
@@ -1001,7 +1010,24 @@ __device__ __forceinline__ void classify_one(const JetDev &d, bool strong, u32 s
     {
         bool sure = false, boundary = false;
         u64 key = 0;
-        if (valid) {
+        if (valid && d.race) {
+            // the offset came with the minibucket add: the score is known here;
+            // strong candidates still need the cookie cutter (resolve launch)
+            const u32 g = d.gid[u], ub = d.work[u];
+            const s64 limit = d.bw[ub] - d.lmax;
+            const s64 sc = bucket_prefix(d, g, ub) + d.gain1[u];
+            if (sc < limit) {
+                if (!strong) {
+                    sure = true;
+                    d.mark[u] = stamp;
+                    d.oldp[u] = ub;
+                } else {
+                    boundary = true;
+                    key = ((u64) g << 32) | u;
+                    if (sc + vwt(d, u) >= limit) d.evict_adjust[ub] = sc + vwt(d, u);
+                }
+            }
+        } else if (valid) {
             const u32 g = d.gid[u], ub = d.work[u];
             const s64 limit = d.bw[ub] - d.lmax;
             if (bucket_prefix(d, g, ub) < limit) {
@@ -1503,6 +1529,45 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
     return true;
 }
 
+// Racing mode, strong rebalancing: the list holds every taken candidate with
+// its score known; the cookie cutter, then each candidate's chunk, and an
+// overflowing candidate's next position from a fetch-add on the overflow
+// counter (jet:576-583) in whatever order the lanes get there.
+__device__ void resolve_race(const JetDev &d, u32 stamp, u32 L, s64 *s_es) {
+    if (threadIdx.x < WAVE) cookie_cutter(d, max_block_weight(d, true));
+    __syncthreads();
+    const bool es_lds = d.k + 1 <= ES_CAP;
+    if (es_lds)
+        for (u32 i = threadIdx.x; i <= d.k; i += RB) s_es[i] = d.evict_start[i];
+    __syncthreads();
+    const s64 *es = es_lds ? s_es : d.evict_start;
+    for (u32 i0 = 0; i0 < L; i0 += RB) {
+        const u32 i = i0 + threadIdx.x;
+        u32 u = 0;
+        if (i < L) {
+            const u64 key = d.rlist[i];
+            u = (u32) key;
+            const u32 g = (u32) (key >> 32), ub = d.work[u];
+            const s64 wu = vwt(d, u);
+            s64 pos = bucket_prefix(d, g, ub) + d.gain1[u] + d.evict_adjust[ub];
+            u32 dst = ub;
+            while (true) {
+                const s64 id = chunk_index(es, d.k, pos);
+                if (id >= 0 && id < (s64) d.k && wu / 2 <= es[id + 1] - pos) {
+                    dst = (u32) id;
+                    break;
+                }
+                if (id >= (s64) d.k) break;
+                pos = (s64) atomicAdd((unsigned long long *) d.counter, (unsigned long long) wu);
+            }
+            d.dest[u] = dst;
+            d.mark[u] = stamp;
+            d.oldp[u] = ub;
+        }
+        append(i < L, u, d.moves_x, &d.ctl->st[ST_MOVES]);
+    }
+}
+
 template<u32 CAP>
 __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
     __shared__ __align__(16) char smem[resolve_lds_bytes(CAP)];
@@ -1510,6 +1575,10 @@ __global__ __launch_bounds__(RB) void resolve_kernel(JetDev d) {
     const CtlHead hd = ctl_head(ctl);
     const u32 L = (u32) ctl->st[ST_RLIST];  // the same cache line as the head
     if (!live(hd) || hd.mode == 0) return;
+    if (d.race) {
+        if (hd.mode == 2 && L) resolve_race(d, hd.stamp, L, (s64 *) smem);
+        return;
+    }
     const ResolveLds lds = resolve_lds(smem, CAP);
     const bool ok = resolve_body<CAP>(d, hd.mode == 2, hd.stamp, L, lds, nullptr);
     if (!ok && threadIdx.x == 0) ctl->stalled = 1;
@@ -2793,6 +2862,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     d.vw = g.vw;
     d.hub_deg = meta.hub_deg;
     d.D = D;
+    d.race = c.race ? 1u : 0u;
     d.count_bytes = c.prof_on ? 1u : 0u;
     if (D) ultra = false;  // ProMap_jet_refine runs one filter ratio, which its LP does not read
     d.part = part;
@@ -2862,8 +2932,8 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         h.bm_lds = std::max(bm_sweep_lds_bytes(k, h.bm_groups), resolve_lds_bytes(BM_RCAP));
         // one workgroup walks every edge each iteration: dense small levels
         // (coarse levels of skewed graphs) stay on the multi-launch form
-        h.block_mode = n <= c.block_jet_max_n && (u64) g.m <= 32ull * c.block_jet_max_n && meta.n_hubs == 0 &&
-                       h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
+        h.block_mode = !c.race && n <= c.block_jet_max_n && (u64) g.m <= 32ull * c.block_jet_max_n &&
+                       meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
         if (h.block_mode) {
             h.bm_state_off = (u32) ((h.bm_lds + 15) & ~(size_t) 15);
             h.bm_ls = c.block_lds && h.bm_state_off + bm_state_bytes(n, k) <= c.max_lds;
@@ -2875,7 +2945,9 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         const u64 per_copy = 8ull * ((u64) d.t + 1);
         const u64 wgs = (u64) div_up(n, 256);
         u32 s = 1;
-        if (c.bucket_copies)
+        if (c.race)  // one copy: the fetch-add results are offsets in the minibucket
+            s = 1;
+        else if (c.bucket_copies)
             while (!h.block_mode && s * 2 <= c.bucket_copies) s *= 2;
         else
             while (!h.block_mode && s < 32 && (u64) s * 2 * 64 <= wgs && (u64) s * 2 * per_copy <= (8ull << 20)) s *= 2;

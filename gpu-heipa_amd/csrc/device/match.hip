@@ -155,6 +155,11 @@ __global__ __launch_bounds__(MB) void pick_kernel(DGraph g, u32 *__restrict__ vc
     }
 }
 
+__device__ __forceinline__ void wave_add_u64(unsigned long long *ctr, u64 v) {
+    v = wave_sum(v);
+    if (lane_id() == 0 && v) atomicAdd(ctr, (unsigned long long) v);
+}
+
 // Repick and compaction (two_hop_matching.h:337-371).  Phase A: one thread
 // per list entry tests whether the vertex is unmatched with a matched
 // candidate and lists it in LDS; phase B: G lanes per listed vertex pick its
@@ -167,7 +172,8 @@ constexpr u32 RP_KEEP = 2048;
 template<int G, bool UEW>
 __global__ __launch_bounds__(MB) void repick_kernel(DGraph g, u32 *__restrict__ vcmap, u32 *__restrict__ hn, u32 seed,
                                                     const u32 *__restrict__ list, const u32 *__restrict__ len_ptr,
-                                                    u32 hub_deg, u32 *__restrict__ out_list, u32 *__restrict__ out_len) {
+                                                    u32 hub_deg, u32 *__restrict__ out_list, u32 *__restrict__ out_len,
+                                                    unsigned long long *deg_acc) {
     __shared__ u32 s_work[MB];
     __shared__ u32 s_keep[RP_KEEP];
     __shared__ u32 s_nwork, s_nkeep, s_base;
@@ -208,18 +214,21 @@ __global__ __launch_bounds__(MB) void repick_kernel(DGraph g, u32 *__restrict__ 
         }
         __syncthreads();
         const u32 nw = s_nwork;
+        u32 degs = 0;  // summed degree of this workgroup's repicked vertices (byte model, profiling only)
         for (u32 r0 = 0; r0 < nw; r0 += MB / G) {
             const u32 idx = r0 + gi;
             const bool act = idx < nw;
             const u32 w = act ? s_work[idx] : 0;
             Cand c{0, 0, SENT};
             if (act) c = pick_partial<false, UEW>(g, vcmap, g.xadj[w], g.xadj[w + 1], lg, G, xorshift32(w ^ seed));
+            if (deg_acc && act && lg == 0) degs += g.xadj[w + 1] - g.xadj[w];
             c = cand_reduce<G>(c);
             if (act && lg == 0) {  // only this group reads or writes entry idx
                 hn[w] = c.v;
                 s_work[idx] = c.v;  // the new candidate, at the vertex's list position
             }
         }
+        if (deg_acc) wave_add_u64(deg_acc, degs);
         __syncthreads();
         if (need) h = s_work[pos];
         const bool keep = unmatched && h != SENT;
@@ -260,7 +269,10 @@ __global__ __launch_bounds__(MB) void pick_hub_kernel(DGraph g, u32 *__restrict_
     }
 }
 
-// commit sub-round, claim phase (p1 + p2 of two_hop_matching.h:313-329)
+// commit sub-round, claim phase (p1 + p2 of two_hop_matching.h:313-329);
+// RACE: the first proposer to claim a receiver wins (the reference's CAS,
+// :321-329), else the smallest proposer id (rule D1)
+template<bool RACE>
 __global__ void claim_kernel(u32 n, const u32 *__restrict__ vcmap, const u32 *__restrict__ hn, u32 *__restrict__ claim,
                              const u32 *__restrict__ list, const u32 *__restrict__ len_ptr, u32 r) {
     const u32 len = list ? *len_ptr : n;
@@ -273,7 +285,8 @@ __global__ void claim_kernel(u32 n, const u32 *__restrict__ vcmap, const u32 *__
         if (vcmap[v] != SENT) continue;
         const u32 hv = hn[v];
         if (hv == SENT || proposes(v, hv, r)) continue;
-        atomicMin(&claim[v], u);
+        if (RACE) atomicCAS(&claim[v], SENT, u);
+        else atomicMin(&claim[v], u);
     }
 }
 
@@ -448,13 +461,19 @@ __global__ void propagate_kernel(const u32 *vcmap, u32 n, u32 *mapping) {
     }
 }
 
+// Algorithmic bytes (the pick pass over an active set A: 8 #A + 4 deg(A)(1+w)
+// + 4 #A (hn) + 4 #N(A) (vcmap), w = 1 with edge weights): the initial pick
+// reads every offset and edge once and writes hn, 4(n+1) + 4m(1+w) + 4n; the
+// repick and claim / resolve passes are charged after each batch from the
+// read-back list lengths (and, when profiling, the repicked degrees).
 template<int G>
 void launch_pick(Ctx &c, const DGraph &g, const GraphMeta &meta, bool initial, u32 *vcmap, u32 *hn, u32 seed,
-                 const u32 *list, const u32 *len_ptr, u32 len_bound, u32 *out_list, u32 *out_len) {
+                 const u32 *list, const u32 *len_ptr, u32 len_bound, u32 *out_list, u32 *out_len,
+                 unsigned long long *deg_acc) {
     const bool uew = g.ew == nullptr;
     const dim3 block(MB);
-    const double bytes = 0.0;
     if (initial) {
+        const double bytes = 4.0 * (g.n + 1) + 4.0 * g.m * (uew ? 1 : 2) + 4.0 * g.n;
         const dim3 grid(grid_for(c, len_bound, MB / G, 8));
         if (uew)
             launch(c, "match_initial_pick", bytes, grid, block, 0, pick_kernel<G, true, true>, g, vcmap, hn, seed,
@@ -465,11 +484,11 @@ void launch_pick(Ctx &c, const DGraph &g, const GraphMeta &meta, bool initial, u
     } else {
         const dim3 grid(grid_for(c, len_bound, MB, 2));
         if (uew)
-            launch(c, "match_repick", bytes, grid, block, 0, repick_kernel<G, true>, g, vcmap, hn, seed, list, len_ptr,
-                   meta.hub_deg, out_list, out_len);
+            launch(c, "match_repick", 0.0, grid, block, 0, repick_kernel<G, true>, g, vcmap, hn, seed, list, len_ptr,
+                   meta.hub_deg, out_list, out_len, deg_acc);
         else
-            launch(c, "match_repick", bytes, grid, block, 0, repick_kernel<G, false>, g, vcmap, hn, seed, list,
-                   len_ptr, meta.hub_deg, out_list, out_len);
+            launch(c, "match_repick", 0.0, grid, block, 0, repick_kernel<G, false>, g, vcmap, hn, seed, list,
+                   len_ptr, meta.hub_deg, out_list, out_len, deg_acc);
     }
 }
 
@@ -541,12 +560,21 @@ u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping
     const dim3 blk(256);
 
     // initial pick
+    // profiling: summed degree of the repicked vertices, for the byte model
+    unsigned long long *deg_acc = nullptr;
+    if (c.prof_on) {
+        deg_acc = bs.get<unsigned long long>(1);
+        fill<unsigned long long>(c, deg_acc, 1, 0ull);
+    }
+    const double wb = g.ew ? 2.0 : 1.0;
     if (g.ew == nullptr) {
-        launch(c, "match_initial_pick", 0.0, dim3(grid_for(c, n, 256)), blk, 0, initial_pick_uniform_kernel, g.xadj,
-               g.adj, n, seed, hn);
+        // offsets, one gathered neighbour and hn per vertex
+        launch(c, "match_initial_pick", 4.0 * (n + 1) + 8.0 * n, dim3(grid_for(c, n, 256)), blk, 0,
+               initial_pick_uniform_kernel, g.xadj, g.adj, n, seed, hn);
     } else {
         HEIPA_DISPATCH_G(meta.G, launch_pick, c, g, meta, true, vcmap, hn, seed, (const u32 *) nullptr,
-                         (const u32 *) nullptr, n, (u32 *) nullptr, (u32 *) nullptr);
+                         (const u32 *) nullptr, n, (u32 *) nullptr, (u32 *) nullptr,
+                         (unsigned long long *) nullptr);
         if (meta.n_hubs)
             launch(c, "match_initial_pick", 0.0, dim3(meta.n_hubs), dim3(MB), 0, pick_hub_kernel<true, false>, g,
                    vcmap, hn, seed, (const u32 *) meta.hubs);
@@ -566,8 +594,8 @@ u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping
             const u32 round_seed = seed ^ (round * 0x9e3779b1u);
             const dim3 grid(grid_for(c, len_bound, 256, 8));
             for (u32 r = 0; r < 4; ++r) {
-                launch(c, "match_claim", 0.0, grid, blk, 0, claim_kernel, n, (const u32 *) vcmap, (const u32 *) hn,
-                       claim, cur_list, cur_len, r);
+                launch(c, "match_claim", 0.0, grid, blk, 0, c.race ? claim_kernel<true> : claim_kernel<false>, n,
+                       (const u32 *) vcmap, (const u32 *) hn, claim, cur_list, cur_len, r);
                 launch(c, "match_resolve", 0.0, grid, blk, 0, resolve_kernel, n, vcmap, claim, cur_list, cur_len);
             }
             if (meta.n_hubs)
@@ -578,7 +606,7 @@ u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping
             u32 *next_len = lens + (round % 64);
             fill<u32>(c, next_len, 1, 0u);
             HEIPA_DISPATCH_G(meta.G, launch_pick, c, g, meta, false, vcmap, hn, round_seed, cur_list, cur_len,
-                             len_bound, next_list, next_len);
+                             len_bound, next_list, next_len, deg_acc);
             cur_list = next_list;
             cur_len = next_len;
         }
@@ -588,8 +616,18 @@ u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping
         c.sync();
         u32 len = 0;
         bool done = false;
+        u32 in_len = len_bound;  // input list length of round first_round
         for (u32 rr = first_round; rr < round; ++rr) {
+            // bytes of round rr over its input list of length L: four claim
+            // passes (list, hn and vcmap of u, hn and vcmap of the candidate:
+            // 20 L), four resolve passes (list, claim read and reset: 12 L), the
+            // repick's phase A (list, vcmap, hn, vcmap of the candidate: 16 L)
+            const double L = (double) in_len;
+            prof_bytes(c, "match_claim", 80.0 * L);
+            prof_bytes(c, "match_resolve", 48.0 * L);
+            prof_bytes(c, "match_repick", 16.0 * L);
             len = pinned[rr - first_round];
+            in_len = len;
             if (len == 0) {
                 exact_rounds = rr + 1;  // the loop of the reference stops after this round
                 done = true;
@@ -602,6 +640,11 @@ u32 match(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 seed, u32 *mapping
         require(round < 20000, "matching did not converge (is the graph symmetric?)");
     }
     st.rounds = exact_rounds;
+    if (deg_acc) {  // the repicked vertices' edges (and weights) and their neighbours' vcmap
+        unsigned long long dsum = 0;
+        readback(c, &dsum, deg_acc, 1);
+        prof_bytes(c, "match_repick", 4.0 * wb * (double) dsum + 4.0 * std::min<double>((double) dsum, n));
+    }
 
     u32 *scr = bs.get<u32>(4);
     u32 unmapped = count_unmapped(c, vcmap, n, scr);

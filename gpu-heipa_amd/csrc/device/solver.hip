@@ -367,6 +367,7 @@ void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u
     const double imb = adaptive_imbalance(p.eps, p.gW, p.gk, g.gw, p.k_rem[l - 1 - ident.size()], l - ident.size());
     if (k == 1) fill<u32>(c, tmp, g.n, 0u);
     else solve(c, g, k, imb, p.seed, p.ultra, tmp, nullptr);
+    c.hkey = 0;  // the caller's graph key names the input graph only, not the children
     if (ident.size() == l - 1) {
         u32 offset = 0;
         for (u32 i = 0; i + 1 < l; ++i) offset += ident[i] * p.index_vec[p.index_vec.size() - 1 - i];
@@ -392,12 +393,108 @@ void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u
     c.arena.release_front(depth0);
 }
 
+// ---- resident hierarchies (HierCache): the entry of the context's graph key
+std::shared_ptr<HierEntry> hier_entry(Ctx &c, const DGraph &g0) {
+    if (!c.hcache || !c.hkey) return nullptr;
+    HierCache &hc = *c.hcache;
+    std::lock_guard<std::mutex> lk(hc.m);
+    std::shared_ptr<HierEntry> &e = hc.entries[c.hkey];
+    if (!e || e->n0 != g0.n || e->m0 != g0.m || e->gw0 != g0.gw) {  // a new graph under this key
+        if (e) hc.bytes -= e->bytes;
+        e = std::make_shared<HierEntry>();
+        e->n0 = g0.n;
+        e->m0 = g0.m;
+        e->gw0 = g0.gw;
+    }
+    e->last_use = ++hc.clock;
+    return e;
+}
+
+bool hier_level(HierEntry &e, size_t l, HierLevel &out) {
+    std::lock_guard<std::mutex> lk(e.m);
+    if (l >= e.levels.size()) return false;
+    out = e.levels[l];
+    return true;
+}
+
+// Copies coarsening step l (the map of level l, the graph of level l + 1)
+// into device memory the cache owns, unless another solve appended it first;
+// then evicts least recently used entries no solve holds while the cache is
+// over its budget.  No device memory for it: the step is not kept.
+void hier_append(Ctx &c, HierEntry &e, size_t l, u32 fine_n, const u32 *map, const DGraph &cg) {
+    {
+        std::lock_guard<std::mutex> lk(e.m);
+        if (e.levels.size() != l) return;
+    }
+    auto al = [](size_t b) { return (b + 255) & ~(size_t) 255; };
+    const size_t b_map = al(4ull * fine_n), b_x = al(4ull * (cg.n + 1)), b_a = al(4ull * cg.m);
+    const size_t b_v = al(4ull * cg.n);
+    const size_t nb = b_map + b_x + 2 * b_a + b_v;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, nb) != hipSuccess) {
+        (void) hipGetLastError();
+        return;
+    }
+    char *p = (char *) mem;
+    HierLevel lv;
+    lv.fine_n = fine_n;
+    lv.map = (u32 *) p;
+    p += b_map;
+    u32 *x = (u32 *) p;
+    p += b_x;
+    u32 *a = (u32 *) p;
+    p += b_a;
+    s32 *w = (s32 *) p;
+    p += b_a;
+    s32 *v = (s32 *) p;
+    copy_d2d(c, lv.map, map, 4ull * fine_n);
+    copy_d2d(c, x, cg.xadj, 4ull * (cg.n + 1));
+    if (cg.m) copy_d2d(c, a, cg.adj, 4ull * cg.m);
+    if (cg.m && cg.ew) copy_d2d(c, w, cg.ew, 4ull * cg.m);
+    if (cg.vw) copy_d2d(c, v, cg.vw, 4ull * cg.n);
+    c.sync();
+    lv.g = cg;
+    lv.g.xadj = x;
+    lv.g.adj = a;
+    lv.g.ew = cg.ew ? w : nullptr;
+    lv.g.vw = cg.vw ? v : nullptr;
+    {
+        std::lock_guard<std::mutex> lk(e.m);
+        if (e.levels.size() != l) {
+            (void) hipFree(mem);
+            return;
+        }
+        e.levels.push_back(lv);
+        e.mem.push_back(mem);
+        e.bytes += nb;
+    }
+    HierCache &hc = *c.hcache;
+    std::vector<std::shared_ptr<HierEntry>> dropped;  // destroyed (freed) outside the lock
+    {
+        std::lock_guard<std::mutex> lk(hc.m);
+        hc.bytes += nb;
+        hc.levels_built++;
+        while (hc.bytes > hc.budget) {
+            auto victim = hc.entries.end();
+            for (auto it = hc.entries.begin(); it != hc.entries.end(); ++it)
+                if (it->second.use_count() == 1 && (victim == hc.entries.end() ||
+                                                    it->second->last_use < victim->second->last_use))
+                    victim = it;
+            if (victim == hc.entries.end()) break;  // every entry is in use
+            hc.bytes -= victim->second->bytes;
+            dropped.push_back(std::move(victim->second));
+            hc.entries.erase(victim);
+        }
+    }
+}
+
 } // namespace
 
 SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, bool ultra, u32 *part,
                   std::vector<heipa_level_trace> *levels, const MapSpec *map) {
     SolveResult res;
     require(k >= 1, "k must be >= 1");
+    c.prof_l0 = false;
     if (map) ultra = false;
     res.lmax = (s64) std::ceil((1.0 + imbalance) * ((double) g0.gw / (double) k));
     const size_t depth0 = c.arena.front_depth();
@@ -409,20 +506,35 @@ SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, b
     std::vector<u32 *> maps;
     const u32 max_n = 8 * k;
     double t0 = now_ms();
+    std::shared_ptr<HierEntry> ent = hier_entry(c, g0);
     while (graphs.back().n > max_n) {
+        const size_t l = graphs.size() - 1;
+        HierLevel lv;
+        if (ent && hier_level(*ent, l, lv)) {  // a resident level of this graph
+            maps.push_back(lv.map);
+            graphs.push_back(lv.g);
+            metas.push_back(make_meta(c, lv.g, false));
+            std::lock_guard<std::mutex> lk(c.hcache->m);
+            c.hcache->levels_reused++;
+            continue;
+        }
         const DGraph &g = graphs.back();
         u32 *map = (u32 *) c.arena.push_front(4ull * g.n);
         double ta = now_ms();
+        c.prof_l0 = c.prof_on && graphs.size() == 1;
         const u32 nc = match(c, g, metas.back(), 12345u, map, nullptr);
         c.sync();
         double tb = now_ms();
         res.ms[0] += tb - ta;
         CoarseGraph cg = contract(c, g, metas.back(), map, nc);
+        c.prof_l0 = false;
+        const u32 fine_n = g.n;
         maps.push_back(map);
         graphs.push_back(cg.g);
         metas.push_back(make_meta(c, cg.g, false));
         c.sync();
         res.ms[1] += now_ms() - tb;
+        if (ent) hier_append(c, *ent, l, fine_n, map, cg.g);
     }
     (void) t0;
     // initial partition of the coarsest graph on the host
@@ -474,8 +586,10 @@ SolveResult solve(Ctx &c, const DGraph &g0, u32 k, double imbalance, u64 seed, b
         c.sync();
         double tb = now_ms();
         res.ms[3] += tb - ta;
+        c.prof_l0 = c.prof_on && l == 0;
         JetResult jr = jet_refine(c, graphs[l], metas[l], cur, k, res.lmax, ultra, (u32) l, cut, maxw, nullptr,
                                   map ? map->D : nullptr);
+        c.prof_l0 = false;
         cut = jr.cut;
         maxw = jr.maxw;
         res.jet_iterations += jr.iterations;
@@ -518,6 +632,11 @@ void hm_partition(Ctx &c, const DGraph &g, const std::vector<u32> &hier, double 
     u32 *n_to_o = (u32 *) c.arena.push_front(4ull * (g.n ? g.n : 1));
     launch(c, "hm_init", 4.0 * g.n, dim3(grid_for(c, g.n, 256)), dim3(256), 0, iota_kernel<u32>, n_to_o, (u64) g.n);
     std::vector<u32> ident;
+    struct KeyRestore {
+        Ctx &c;
+        u64 key;
+        ~KeyRestore() { c.hkey = key; }
+    } kr{c, c.hkey};
     hm_rec(c, g, n_to_o, l - 1, ident, p, part);
     c.sync();
     c.arena.release_front(depth0);

@@ -1,10 +1,12 @@
 // Batched and served forms: a pool of workers on one device.
 //
 // Each worker owns a heipa_ctx (its own HIP stream, arena and pinned staging)
-// and a host thread.  Jobs wait in one FIFO queue; the head job starts when a
-// worker is free and the device-memory estimates of the running jobs plus its
-// own stay within the pool's budget (admission by estimate, so requests are
-// admitted as workers and memory free up).  A job runs exactly the
+// and a host thread.  Jobs wait in one queue in arrival order; a free worker
+// takes the first queued job whose device-memory estimate fits beside the
+// running jobs' within the pool's budget (admission by estimate, so requests
+// are admitted as workers and memory free up).  A job that does not fit yet
+// may be passed by later, smaller ones, at most MAX_PASSED times: after that
+// nothing behind it starts before it, so it cannot starve.  A job runs exactly the
 // single-graph entry point (heipa_partition / heipa_hm_partition) on its
 // worker's context: its result is the same alone and in the mix, and the
 // kernels of jobs on different workers overlap on the device.
@@ -27,7 +29,11 @@ struct heipa_pool {
     std::vector<std::thread> threads;
     std::mutex m;
     std::condition_variable cv_work, cv_done;
-    std::deque<heipa_job *> queue;
+    struct Queued {
+        heipa_job *job;
+        uint32_t passed;  // later jobs admitted before it
+    };
+    std::deque<Queued> queue;
     uint64_t inflight = 0;
     bool stop = false;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -81,24 +87,44 @@ void run_job(heipa_ctx *ctx, heipa_job &j) {
     j.status = heipa_block_weights(ctx, &j.graph, j.part, k, bw.data(), &j.maxw);
 }
 
+constexpr uint32_t MAX_PASSED = 16;
+
+// index of the job a free worker takes now, or -1: the first queued job that
+// fits the budget, unless an earlier one has been passed MAX_PASSED times
+long admissible(const heipa_pool *p) {
+    for (size_t i = 0; i < p->queue.size(); ++i) {
+        const heipa_pool::Queued &q = p->queue[i];
+        if (p->inflight + q.job->estimate <= p->budget) return (long) i;
+        if (q.passed >= MAX_PASSED) return -1;
+    }
+    return -1;
+}
+
 void worker(heipa_pool *p, uint32_t slot) {
     for (;;) {
         heipa_job *job = nullptr;
         {
             std::unique_lock<std::mutex> lk(p->m);
+            long pick = -1;
             p->cv_work.wait(lk, [&] {
-                return (p->stop && p->queue.empty()) ||
-                       (!p->queue.empty() && p->inflight + p->queue.front()->estimate <= p->budget);
+                if (p->stop && p->queue.empty()) return true;
+                pick = admissible(p);
+                return pick >= 0;
             });
-            if (p->queue.empty()) return;  // stopping
-            job = p->queue.front();
-            p->queue.pop_front();
+            if (pick < 0) return;  // stopping
+            job = p->queue[(size_t) pick].job;
+            for (long i = 0; i < pick; ++i) p->queue[(size_t) i].passed++;
+            p->queue.erase(p->queue.begin() + pick);
             p->inflight += job->estimate;
             job->state = 2;
             job->slot = (int32_t) slot;
             job->t_start = p->now();
         }
+        heipa_arena_reset_peak(p->ctxs[slot]);
+        if (job->graph_key) heipa_ctx_set_option(p->ctxs[slot], HEIPA_OPT_GRAPH_KEY, job->graph_key);
         run_job(p->ctxs[slot], *job);
+        if (job->graph_key) heipa_ctx_set_option(p->ctxs[slot], HEIPA_OPT_GRAPH_KEY, 0);
+        job->peak = heipa_arena_peak(p->ctxs[slot]);
         {
             std::lock_guard<std::mutex> lk(p->m);
             p->inflight -= job->estimate;
@@ -173,6 +199,7 @@ int heipa_pool_submit(heipa_pool *p, heipa_job *job) {
     if (p->stop) return HEIPA_ERR_INVALID;
     job->slot = -1;
     job->cut = job->maxw = 0;
+    job->peak = 0;
     job->t_submit = p->now();
     job->t_start = job->t_done = 0.0;
     job->estimate = estimate(*job);
@@ -185,7 +212,7 @@ int heipa_pool_submit(heipa_pool *p, heipa_job *job) {
     }
     job->status = HEIPA_OK;
     job->state = 1;
-    p->queue.push_back(job);
+    p->queue.push_back({job, 0});
     p->cv_work.notify_all();
     return HEIPA_OK;
 }
@@ -216,11 +243,23 @@ double heipa_pool_clock(const heipa_pool *p) { return p ? p->now() : 0.0; }
 int heipa_pool_set_option(heipa_pool *p, int option, uint64_t value) {
     if (!p) return HEIPA_ERR_INVALID;
     std::lock_guard<std::mutex> lk(p->m);
+    if (option == HEIPA_OPT_HIER_CACHE) {  // one cache, shared by every worker
+        int rc = heipa_ctx_set_option(p->ctxs[0], option, value);
+        for (size_t i = 1; rc == HEIPA_OK && i < p->ctxs.size(); ++i)
+            rc = heipa_ctx_share_hier_cache(p->ctxs[i], p->ctxs[0]);
+        return rc;
+    }
     for (heipa_ctx *c: p->ctxs) {
         const int rc = heipa_ctx_set_option(c, option, value);
         if (rc != HEIPA_OK) return rc;
     }
     return HEIPA_OK;
+}
+
+int heipa_pool_hier_cache_stats(const heipa_pool *p, uint64_t *entries, uint64_t *bytes, uint64_t *levels_reused,
+                                uint64_t *levels_built) {
+    if (!p || p->ctxs.empty()) return HEIPA_ERR_INVALID;
+    return heipa_hier_cache_stats(p->ctxs[0], entries, bytes, levels_reused, levels_built);
 }
 
 } // extern "C"

@@ -28,11 +28,15 @@ OPT_BUCKET_COPIES = 5
 OPT_BLOCK_LDS = 6
 OPT_HM_WORKERS = 7
 OPT_HM_WORKER_BYTES = 8
+OPT_RACE = 9
+OPT_HIER_CACHE = 10
+OPT_GRAPH_KEY = 11
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
 s32p = ctypes.POINTER(ctypes.c_int32)
 s64p = ctypes.POINTER(ctypes.c_int64)
+u64p = ctypes.POINTER(ctypes.c_uint64)
 
 
 class HeipaError(RuntimeError):
@@ -72,7 +76,7 @@ class Job(ctypes.Structure):
                 ("status", ctypes.c_int32), ("slot", ctypes.c_int32), ("cut", ctypes.c_int64),
                 ("maxw", ctypes.c_int64), ("estimate", ctypes.c_uint64), ("t_submit", ctypes.c_double),
                 ("t_start", ctypes.c_double), ("t_done", ctypes.c_double), ("state", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("reserved", ctypes.c_uint32), ("peak", ctypes.c_uint64), ("graph_key", ctypes.c_uint64)]
 
 
 class LevelTrace(ctypes.Structure):
@@ -113,6 +117,10 @@ def lib():
             "heipa_arena_mark": (c_u64, [vp]),
             "heipa_arena_release": (c_int, [vp, c_u64]),
             "heipa_arena_peak": (c_u64, [vp]),
+            "heipa_arena_reset_peak": (c_int, [vp]),
+            "heipa_ctx_share_hier_cache": (c_int, [vp, vp]),
+            "heipa_hier_cache_stats": (c_int, [vp, u64p, u64p, u64p, u64p]),
+            "heipa_pool_hier_cache_stats": (c_int, [vp, u64p, u64p, u64p, u64p]),
             "heipa_download": (c_int, [vp, vp, vp, c_u64]),
             "heipa_upload": (c_int, [vp, vp, vp, c_u64]),
             "heipa_edge_cut": (c_int, [vp, GP, vp, s64p]),
@@ -533,6 +541,12 @@ class Context:
     def set_option(self, option, value):
         self._check("heipa_ctx_set_option", self.L.heipa_ctx_set_option(self.h, int(option), int(value)))
 
+    def hier_cache_stats(self):
+        """(graphs held, device bytes, levels reused, levels built) of the context's hierarchy cache"""
+        v = [ctypes.c_uint64() for _ in range(4)]
+        self._check("heipa_hier_cache_stats", self.L.heipa_hier_cache_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     # per-kernel timing
     def prof_enable(self, on=True):
         self._check("heipa_prof_enable", self.L.heipa_prof_enable(self.h, int(on)))
@@ -568,9 +582,10 @@ class Pool:
         self._keep = {}
 
     @staticmethod
-    def job(g: "DeviceGraph", k=0, hier=None, imbalance=0.03, seed=1, ultra=False, part=None):
+    def job(g: "DeviceGraph", k=0, hier=None, imbalance=0.03, seed=1, ultra=False, part=None, graph_key=0):
         """A job for g (partition into k blocks, or HM with hierarchy `hier`); the
-        device partition tensor is allocated unless given."""
+        device partition tensor is allocated unless given; graph_key names g for
+        resident hierarchies (OPT_HIER_CACHE)."""
         torch = _torch()
         j = Job()
         j.graph = g.desc
@@ -581,6 +596,7 @@ class Pool:
         else:
             j.kind, j.k = JOB_PARTITION, k
         j.ultra, j.imbalance, j.seed = int(ultra), imbalance, seed
+        j.graph_key = graph_key
         if part is None:
             part = torch.empty(max(g.n, 1), dtype=torch.int32, device=g.xadj.device)
         j.part = part.data_ptr()
@@ -616,6 +632,14 @@ class Pool:
 
     def clock(self):
         return self.L.heipa_pool_clock(self.h)
+
+    def hier_cache_stats(self):
+        """(graphs held, device bytes, levels reused, levels built) of the shared hierarchy cache"""
+        v = [ctypes.c_uint64() for _ in range(4)]
+        rc = self.L.heipa_pool_hier_cache_stats(self.h, *[ctypes.byref(x) for x in v])
+        if rc != OK:
+            raise HeipaError("heipa_pool_hier_cache_stats", rc)
+        return tuple(x.value for x in v)
 
     def set_option(self, option, value):
         rc = self.L.heipa_pool_set_option(self.h, int(option), int(value))

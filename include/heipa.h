@@ -89,7 +89,8 @@ const char *heipa_ctx_last_error(const heipa_ctx *ctx);
 void *heipa_ctx_stream(heipa_ctx *ctx);          /* the hipStream_t all work runs on */
 uint64_t heipa_arena_mark(const heipa_ctx *ctx); /* front-stack depth, for heipa_arena_release */
 int heipa_arena_release(heipa_ctx *ctx, uint64_t mark);
-uint64_t heipa_arena_peak(const heipa_ctx *ctx); /* peak bytes in use */
+uint64_t heipa_arena_peak(const heipa_ctx *ctx); /* peak bytes in use since creation or the last reset */
+int heipa_arena_reset_peak(heipa_ctx *ctx);      /* the peak restarts from the bytes in use now */
 /* copies between device and host memory on the context stream (synchronous);
  * lets callers without their own HIP runtime handle read results */
 int heipa_download(heipa_ctx *ctx, void *host_dst, const void *dev_src, uint64_t bytes);
@@ -213,6 +214,8 @@ typedef struct {
     double t_submit, t_start, t_done; /* ms on the pool's clock (heipa_pool_clock) */
     uint32_t state;           /* 0 new, 1 queued, 2 running, 3 done (read through heipa_pool_wait) */
     uint32_t reserved;
+    uint64_t peak;            /* device bytes the run used at most (its worker's arena peak) */
+    uint64_t graph_key;       /* input: the graph's key for resident hierarchies (HEIPA_OPT_GRAPH_KEY), 0 none */
 } heipa_job;
 int heipa_pool_create(int device, uint32_t slots, uint64_t arena_bytes, uint64_t memory_budget, heipa_pool **out);
 int heipa_pool_destroy(heipa_pool *pool);  /* waits for queued and running jobs */
@@ -227,8 +230,12 @@ int heipa_pool_submit(heipa_pool *pool, heipa_job *job);
 /* waits until the job is done or timeout_ms passes (negative: no limit); 1 done, 0 not yet */
 int heipa_pool_wait(heipa_pool *pool, heipa_job *job, double timeout_ms);
 double heipa_pool_clock(const heipa_pool *pool);
-/* heipa_ctx_set_option on every worker's context (call while no job runs) */
+/* heipa_ctx_set_option on every worker's context (call while no job runs); HEIPA_OPT_HIER_CACHE makes
+ * one cache that all workers share */
 int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
+/* heipa_hier_cache_stats of the pool's shared cache */
+int heipa_pool_hier_cache_stats(const heipa_pool *pool, uint64_t *entries, uint64_t *bytes, uint64_t *levels_reused,
+                                uint64_t *levels_built);
 
 /* ---- context options.  No counterpart in the reference (its sizes are compile-time).
  * HEIPA_OPT_RESOLVE_CAP: most rebalance resolve entries one workgroup sorts in LDS before an
@@ -266,7 +273,29 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
  * calling context, so a small value still gives the same result.  Setting it closes the cached
  * workers. */
 #define HEIPA_OPT_HM_WORKER_BYTES 8
+/* HEIPA_OPT_RACE: 1 resolves the reference's races by atomic order, as its GPU build does (a
+ * matching receiver goes to the first proposer, two_hop_matching.h:321-329; a rebalancing
+ * candidate's offset in its minibucket and a strong overflow position are atomic fetch-add results,
+ * jet_label_propagation.h:439, :684, :580), instead of the deterministic rules (default 0).  Results
+ * then vary from run to run; every partition still meets lmax wherever the deterministic run does. */
+#define HEIPA_OPT_RACE 9
+/* HEIPA_OPT_HIER_CACHE: keep coarse hierarchies in device memory between calls, up to `value` bytes
+ * (least recently used graphs go first; 0 drops the cache).  The hierarchy of a graph does not depend
+ * on k (two_hop_matching.h:590 fixes the matching seed; solver.h:343-347 only stops at n <= 8k), so a
+ * later heipa_partition / heipa_im_partition / heipa_hm_partition (its first split) of the same graph
+ * reuses the levels it needs, builds and keeps the ones beyond them, and returns the partition a fresh
+ * solve returns.  A graph is named by HEIPA_OPT_GRAPH_KEY. */
+#define HEIPA_OPT_HIER_CACHE 10
+/* HEIPA_OPT_GRAPH_KEY: the caller's key of the input graph of the following calls on this context
+ * (0, the default: no reuse).  A key that comes back with another n, m or total weight replaces its
+ * entry; the caller keeps one key per graph. */
+#define HEIPA_OPT_GRAPH_KEY 11
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
+/* dst uses src's hierarchy cache (HEIPA_OPT_HIER_CACHE) from now on: one cache for several contexts */
+int heipa_ctx_share_hier_cache(heipa_ctx *dst, const heipa_ctx *src);
+/* graphs held, device bytes, levels reused and levels built since the cache was created */
+int heipa_hier_cache_stats(const heipa_ctx *ctx, uint64_t *entries, uint64_t *bytes, uint64_t *levels_reused,
+                           uint64_t *levels_built);
 
 /* ---- per-kernel timing (HIP events on the context stream) */
 int heipa_prof_enable(heipa_ctx *ctx, int on);
