@@ -2564,6 +2564,28 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
     }
 }
 
+// LDS the single-workgroup kernel declares itself (its __shared__
+// variables), which the hardware adds to the dynamic size of a launch
+template<int G, bool LS>
+size_t jet_block_static_lds() {
+    hipFuncAttributes fa{};
+    HIP_CHECK(hipFuncGetAttributes(&fa, (const void *) jet_block_kernel<G, LS>));
+    return fa.sharedSizeBytes;
+}
+inline size_t jet_block_static_lds_of(u32 G, bool ls);
+inline size_t jet_block_static_lds(u32 G, bool ls) {
+    return std::max<size_t>(1024, jet_block_static_lds_of(G, ls));  // at least 1 KiB of headroom
+}
+inline size_t jet_block_static_lds_of(u32 G, bool ls) {
+    switch (G) {
+        case 4: return ls ? jet_block_static_lds<4, true>() : jet_block_static_lds<4, false>();
+        case 8: return ls ? jet_block_static_lds<8, true>() : jet_block_static_lds<8, false>();
+        case 16: return ls ? jet_block_static_lds<16, true>() : jet_block_static_lds<16, false>();
+        case 32: return ls ? jet_block_static_lds<32, true>() : jet_block_static_lds<32, false>();
+        default: return ls ? jet_block_static_lds<64, true>() : jet_block_static_lds<64, false>();
+    }
+}
+
 struct RatioTable {
     double r[MAX_RATIOS];
     u32 n;
@@ -2935,8 +2957,12 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         h.block_mode = !c.race && n <= c.block_jet_max_n && (u64) g.m <= 32ull * c.block_jet_max_n &&
                        meta.n_hubs == 0 && h.bm_lds <= std::min<size_t>(c.max_lds, 65536);
         if (h.block_mode) {
+            // a launch's LDS is its dynamic size plus the kernel's own __shared__ variables
             h.bm_state_off = (u32) ((h.bm_lds + 15) & ~(size_t) 15);
-            h.bm_ls = c.block_lds && h.bm_state_off + bm_state_bytes(n, k) <= c.max_lds;
+            h.bm_ls = c.block_lds &&
+                      h.bm_state_off + bm_state_bytes(n, k) + jet_block_static_lds(meta.G, true) <= c.max_lds;
+            require(h.bm_ls || h.bm_lds + jet_block_static_lds(meta.G, false) <= c.max_lds,
+                    "single-workgroup Jet does not fit the LDS");
         }
     }
     // copies of the minibucket totals: about one per 64 sweep workgroups, at

@@ -249,7 +249,9 @@ u64 hm_child_bytes(const DGraph &g, const HMParams &p, u32 level) {
     for (u32 i = 0; i <= level; ++i) k = std::max<u64>(k, p.hier[i]);
     const u64 n = g.n, m = g.m;
     const u64 sections = std::max<u64>(1, std::min<u64>(128, n / (4096 * k) + 1));
-    return 272 * n + 72 * m + 50ull * k * sections * 32 + (64ull << 20);
+    // the pool's measured envelope for an HM job (a worker that runs out
+    // hands its children back to the calling context)
+    return 80 * n + 88 * m + 50ull * k * sections * 32 + (2ull << 20);
 }
 
 // The subtrees below the first split are independent (disjoint vertices,

@@ -180,3 +180,30 @@ def test_jet_large_k(ctx, k, ultra, level, form):
         assert np.array_equal(heipa.to_numpy_u32(pd), op)
 
     _with_form(ctx, form, run)
+
+
+@pytest.mark.parametrize("n", [2300, 2340, 2370, 2371, 2372, 2450])
+def test_jet_block_lds_boundary(ctx, n):
+    """The single-workgroup form keeps a level's state in LDS only if the
+    state, the dynamic work areas and the kernel's own __shared__ variables
+    fit the 160 KiB together (the static part once pushed a launch over the
+    limit).  With the form allowed for any n and k = 8, the state of 2300
+    vertices fits with room, 2370-2372 would have crossed the limit by the
+    kernel's static LDS alone, and 2450 does not fit; every result equals the
+    oracle's."""
+    g = heipa.gen_rgg(n, 8.0, 3, 10)
+    k = 8
+    part0 = random_partition(g.n, k, 2, 1.0)
+    lmax = math.ceil(1.03 * g.n / k)
+    cut0, maxw0 = orc.edge_cut(g, part0), int(block_weights(g, part0, k).max())
+    op, ocut, omaxw, otr = orc.jet_refine(g, part0, k, lmax, False, 0, cut0, maxw0, trace_cap=4096)
+    print(f"jet block boundary n={n}", flush=True)
+
+    def run():
+        pd = heipa.from_numpy_u32(part0)
+        gcut, gmaxw, gtr = ctx.jet_refine(_dev(g), pd, k, lmax, False, 0, cut0, maxw0, trace_cap=4096)
+        _same(f"n{n}", gtr, otr)
+        assert (gcut, gmaxw) == (ocut, omaxw)
+        assert np.array_equal(heipa.to_numpy_u32(pd), op)
+
+    _with_form(ctx, "block-lds", run)

@@ -65,23 +65,30 @@ def test_pool_served_equals_alone(ctx):
         assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
 
 
-def test_pool_admission_budget_serialises(ctx):
-    """A budget that fits one job at a time admits them one by one."""
-    reqs = _requests()[:4]
+def test_pool_admission_respects_budget(ctx):
+    """With a budget of the largest estimate, jobs run together only while
+    their estimates fit it (a smaller job may start before a larger one that
+    waits), the largest job runs alone, and every result equals its alone run."""
+    reqs = _requests()
     probe = heipa.Pool(1, 512 << 20)
     jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs]
-    budget = max(probe.estimate(j) for j in jobs)
+    est = [probe.estimate(j) for j in jobs]
+    budget = max(est)
     probe.close()
     pool = heipa.Pool(4, 512 << 20, budget=budget)
     try:
         pool.run(jobs)
     finally:
         pool.close()
-    spans = sorted((j.t_start, j.t_done) for j in jobs)
-    for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
-        assert b0 >= a1  # no two jobs ran at once
+    for i, ji in enumerate(jobs):
+        running = sum(est[j] for j, jj in enumerate(jobs)
+                      if j != i and jj.t_start <= ji.t_start < jj.t_done)
+        assert running + est[i] <= budget, (i, running, est[i], budget)
+    big = jobs[est.index(budget)]
+    assert not any(j is not big and j.t_start < big.t_done and big.t_start < j.t_done for j in jobs)
     for (dg, s), j in zip(reqs, jobs):
         assert j.status == heipa.OK
+        assert j.estimate >= j.peak > 0
         assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s))
 
 
