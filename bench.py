@@ -9,7 +9,10 @@ the plain drop-in, with p50/p99 latency at equal cut:
     0.5, n log-uniform in [1024, 65536], k from {2,4,8,16,32}, eps 0.03,
     `default`) through the worker pool.  The K timed steps together process
     every graph exactly once (ceil(4096 / K) graphs per step); `value` is
-    their undirected input edges / the time of the K steps.  The plain path
+    their undirected input edges / the time of the K steps.  Before the
+    warmup, the pool's autotuner picks the lane-group caps and the
+    single-workgroup threshold on a sample of 64 of the graphs (launch shapes
+    only: results do not change).  The plain path
     (one context, graphs one after another) is timed on the graphs of the
     first timed step beside it, and every one of those partitions must be
     bit-identical in both forms (equal cut);
@@ -83,6 +86,9 @@ def main():
                     help="HM: worker contexts for the subtrees below the first split (default: the library's)")
     ap.add_argument("--block-jet-n", type=int, default=None,
                     help="largest level run by the single-workgroup Jet (default: the library's)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="headline: skip the pool's launch-shape autotuner (defaults stay)")
+    ap.add_argument("--tune-graphs", type=int, default=64, help="headline: batched graphs the autotuner times")
     ap.add_argument("--batch-count", type=int, default=4096, help="batched: number of graphs")
     ap.add_argument("--plain-sample", type=int, default=256, help="batched: graphs timed through the plain path")
     ap.add_argument("--batch-order", choices=["seq", "size"], default="seq",
@@ -380,6 +386,16 @@ def run_headline(args):
     pool = heipa.Pool(args.slots, arena)
     if args.block_jet_n is not None:
         pool.set_option(heipa.OPT_BLOCK_JET, args.block_jet_n)
+    tune = None
+    if not args.no_autotune:
+        # launch shapes that never change results, timed on an evenly spaced
+        # sample of the graphs before the warmup; the plain path below gets the
+        # same setting, so the speedup is the batching alone
+        tt = time.perf_counter()
+        stride = max(1, len(reqs) // max(1, args.tune_graphs))
+        tune = pool.autotune(jobs_of(list(range(0, len(reqs), stride))[:args.tune_graphs]))
+        tune["seconds"] = round(time.perf_counter() - tt, 2)
+        tune["sample_graphs"] = min(args.tune_graphs, len(range(0, len(reqs), stride)))
     for s in range(W):
         pool.run(jobs_of(batches[s % K]))
     timed = [jobs_of(b) for b in batches]
@@ -430,6 +446,10 @@ def run_headline(args):
     pool.close()
 
     ctx = heipa.Context(0, arena)
+    if tune:
+        ctx.set_option(heipa.OPT_GROUP_CAP, tune["group_cap"])
+        ctx.set_option(heipa.OPT_BLOCK_GROUP_CAP, tune["block_group_cap"])
+        ctx.set_option(heipa.OPT_BLOCK_JET, tune["block_jet"])
     for i in sample[:2]:
         ctx.partition(dgs[i], reqs[i][1]["k"], 0.03, 1, False)
     torch.cuda.synchronize()
@@ -468,13 +488,15 @@ def run_headline(args):
                                f"steps cover every graph once and are submitted back to back",
                    "graphs": len(reqs), "graphs_per_step": per, "undirected_edges": sum(edges),
                    "slots": args.slots, "arena_bytes_per_slot": arena, "input": "device-resident CSR",
+                   "autotune": tune,
                    "mix": {"rgg": sum(1 for r in reqs if r[2] == "rgg"),
                            "rmat": sum(1 for r in reqs if r[2] == "rmat")}},
         "plain": {"value": round(plain_rate, 1), "unit": "input edges/s", "graphs": len(sample),
                   "seconds": round(plain_s, 3), "undirected_edges": es,
                   "batched_value_same_graphs": round(same_graph_batched, 1),
-                  "what": "every 20th batched graph, one after another on one context; the same graphs "
-                          "through the pool give batched_value_same_graphs"},
+                  "what": "every 20th batched graph, one after another on one context (with the "
+                          "autotuned launch shapes); the same graphs through the pool give "
+                          "batched_value_same_graphs"},
         "speedup_vs_plain": round(same_graph_batched / plain_rate, 3),
         "admission_estimate_over_peak": est_ratio,
         "quality": {"failed": failed, "unbalanced": unbalanced,

@@ -187,6 +187,8 @@ struct Ctx {
     size_t max_lds = 65536;  // LDS bytes one workgroup may allocate
     u32 resolve_cap = ~0u;   // HEIPA_OPT_RESOLVE_CAP
     u32 block_jet_max_n = 2048;  // HEIPA_OPT_BLOCK_JET
+    u32 group_cap = 64;         // HEIPA_OPT_GROUP_CAP
+    u32 block_group_cap = 8;    // HEIPA_OPT_BLOCK_GROUP_CAP
     bool use_graphs = true;  // HEIPA_OPT_GRAPHS
     u32 bucket_copies = 0;   // HEIPA_OPT_BUCKET_COPIES (0: automatic)
     bool block_lds = true;   // HEIPA_OPT_BLOCK_LDS

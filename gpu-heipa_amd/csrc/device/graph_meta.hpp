@@ -16,6 +16,12 @@ struct GraphMeta {
     u32 *hubs = nullptr; // device list (arena, owner's lifetime)
 };
 
+// G halved down to at most cap (never below 4)
+inline u32 cap_group(u32 G, u32 cap) {
+    while (G > 4 && G > cap) G >>= 1;
+    return G;
+}
+
 inline u32 group_for_avg(double avg) {
     u32 g = 4;
     while (g < 64 && (double) g < avg) g <<= 1;

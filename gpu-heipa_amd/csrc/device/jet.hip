@@ -1250,7 +1250,8 @@ __global__ __launch_bounds__(JB) void hub_post_kernel(JetDev d, const u32 *hubs)
 // the chunk containing position pos: the last id in [0, k] with es[id] <= pos
 // (es = evict_start, non-decreasing from es[0] = 0; the reference walks it
 // linearly, jet:564-566)
-__device__ __forceinline__ s64 chunk_index(const s64 *es, u32 k, s64 pos) {
+template<typename P>
+__device__ __forceinline__ s64 chunk_index(P es, u32 k, s64 pos) {
     u32 lo = 0, hi = k + 1;
     while (lo < hi) {
         const u32 mid = (lo + hi) >> 1;
@@ -1270,19 +1271,38 @@ __device__ __forceinline__ u32 chunk_of(const s64 *es, u32 k, s64 pos, s64 wu, u
     return ub;
 }
 
-// the serial adjust_scores loop of one overflowing vertex (jet:563-585)
-__device__ __forceinline__ u32 overflow_dest(const s64 *es, u32 k, s64 pos, s64 wu, u32 ub, s64 &counter) {
-    s64 id = 0;
-    while (id < (s64) k) {
+// the serial adjust_scores loop of one overflowing vertex (jet:563-585): its
+// chunk, or k when the vertex stays in its block.  After its own position the
+// vertex takes the counter's positions, counter, counter + wu, ..., until one
+// lands in a chunk with room for wu / 2 (or past the last chunk).  Positions
+// later in a chunk that has no room for the first have less, so the loop
+// jumps to the first position at or past the next chunk's start: the same
+// position and counter as one take at a time, in at most one step per chunk.
+template<typename P>
+__device__ __forceinline__ u32 overflow_chunk(P es, u32 k, s64 pos, s64 wu, s64 &counter) {
+    s64 id = chunk_index(es, k, pos);
+    if (id >= 0 && id < (s64) k && wu / 2 <= es[id + 1] - pos) return (u32) id;
+    if (id >= (s64) k) return k;
+    pos = counter;
+    while (true) {
         id = chunk_index(es, k, pos);
-        if (id >= 0 && id < (s64) k && wu / 2 <= es[id + 1] - pos) return (u32) id;
-        if (id < (s64) k) {
-            pos = counter;
-            counter += wu;
+        if (id >= (s64) k || (id >= 0 && wu / 2 <= es[id + 1] - pos)) break;
+        if (wu <= 0) {  // a weightless vertex never leaves a position before the first chunk
+            id = k;
+            break;
         }
+        const s64 next = es[id + 1];  // > pos; es[0] when id = -1
+        pos += (next - pos + wu - 1) / wu * wu;
     }
-    return ub;
+    counter = pos + wu;
+    return id >= (s64) k ? k : (u32) id;
 }
+
+// The chunk bounds in LDS through an LDS-typed pointer: a pointer that may
+// point to either memory compiles to flat accesses, and the overflow step is
+// a chain of dependent reads.
+typedef __attribute__((address_space(3))) const s64 lds_s64;
+__device__ __forceinline__ lds_s64 *les_of(const s64 *p) { return (lds_s64 *) p; }
 
 // cookie cutter (jet:508-552), by the first wave of the workgroup; on entry
 // evict_adjust holds the filter's values, on exit the eviction position of
@@ -1322,14 +1342,16 @@ struct ResolveLds {
     s64 *soff;
     u8 *ssel;
     s64 *red;    // RB / WAVE
-    u32 *ocount;
+    u32 *ocount; // [0] overflow entries, [1] the first that takes the counter twice
     u32 *olist;  // OVF_CAP
+    u32 *ovid;   // OVF_CAP
+    u32 *oent;   // OVF_CAP
     s64 *s_es;   // ES_CAP
 };
 
 // LDS bytes of ResolveLds for CAP entries (16-byte aligned parts)
 __host__ __device__ constexpr size_t resolve_lds_bytes(u32 cap) {
-    return 16ull * cap + ((cap + 15) & ~15u) + 8ull * (RB / WAVE) + 16 + 4ull * OVF_CAP + 8ull * ES_CAP;
+    return 16ull * cap + ((cap + 15) & ~15u) + 8ull * (RB / WAVE) + 16 + 12ull * OVF_CAP + 8ull * ES_CAP;
 }
 __device__ inline ResolveLds resolve_lds(char *base, u32 cap) {
     ResolveLds r;
@@ -1343,8 +1365,90 @@ __device__ inline ResolveLds resolve_lds(char *base, u32 cap) {
     p += 16;
     r.olist = (u32 *) p;
     p += 4ull * OVF_CAP;
+    r.ovid = (u32 *) p;
+    p += 4ull * OVF_CAP;
+    r.oent = (u32 *) p;
+    p += 4ull * OVF_CAP;
     r.s_es = (s64 *) p;
     return r;
+}
+
+// The strong overflow step (jet:563-585) of oc <= OVF_CAP entries, by the
+// whole workgroup (NT threads, NT >= OVF_CAP).  olist[0, oc) index the resolve
+// arrays; skey holds weight << 32 | vertex and soff the position that did not
+// fit.  Serially, in vertex id order, each entry takes the counter's position
+// and adds its weight, and takes again while that position lands in a chunk
+// without room.  If every entry takes once, the entry of rank r gets the
+// counter plus the weights of the entries ranked before it: ranks come from
+// counting smaller ids, the positions from a scan, and all are checked at
+// once; from the first entry that must take again on, one thread finishes
+// serially.  On exit soff holds each entry's chunk (k: it stays).
+template<int NT, typename P>
+__device__ __forceinline__ void overflow_block(const JetDev &d, P es, const ResolveLds &lds, u32 oc) {
+    static_assert(NT >= (int) OVF_CAP, "one thread per overflow entry");
+    u64 *skey = lds.skey;
+    s64 *soff = lds.soff;
+    const u32 t = threadIdx.x, k = d.k;
+    if (t < oc) lds.ovid[t] = (u32) skey[lds.olist[t]];
+    if (t == 0) lds.ocount[1] = oc;
+    __syncthreads();
+    if (t < oc) {
+        const u32 my = lds.ovid[t];
+        u32 r = 0;
+        for (u32 q = 0; q < oc; ++q) r += lds.ovid[q] < my ? 1u : 0u;  // ids are distinct
+        lds.oent[r] = lds.olist[t];
+    }
+    __syncthreads();
+    const bool act = t < oc;
+    const u32 e = act ? lds.oent[t] : 0;
+    const s64 w = act ? (s64) (skey[e] >> 32) : 0;
+    s64 tot;
+    const s64 ex = block_exclusive_scan<NT>(w, lds.red, tot);
+    const s64 c0 = *d.counter;
+    const s64 pos = c0 + ex;
+    u32 id = k;
+    if (act) {
+        const s64 cid = chunk_index(es, k, pos);
+        if (cid >= 0 && cid < (s64) k && w / 2 <= es[cid + 1] - pos) id = (u32) cid;
+        else if (cid < (s64) k) atomicMin(&lds.ocount[1], t);
+    }
+    __syncthreads();
+    const u32 first = lds.ocount[1];
+    if (act && t < first) soff[e] = id;
+    if (t == first) lds.red[0] = pos;  // the counter when that entry comes
+    __syncthreads();
+    if (t == 0) {
+        // the rest one at a time: each takes the counter (its own position is
+        // known to fail), and since the counter only grows, the chunk search
+        // walks forward from the previous position's chunk
+        s64 counter = first < oc ? lds.red[0] : c0 + tot;
+        s64 cid = first < oc ? chunk_index(es, k, counter) : 0;
+        for (u32 r = first; r < oc; ++r) {
+            const u32 i = lds.oent[r];
+            const s64 wu = (s64) (skey[i] >> 32);
+            s64 pos = counter;
+            u32 got;
+            while (true) {
+                while (cid < (s64) k && es[cid + 1] <= pos) ++cid;
+                if (cid >= (s64) k) {
+                    got = k;
+                    break;
+                }
+                if (cid >= 0 && wu / 2 <= es[cid + 1] - pos) {
+                    got = (u32) cid;
+                    break;
+                }
+                if (wu <= 0) {  // as overflow_chunk
+                    got = k;
+                    break;
+                }
+                pos += (es[cid + 1] - pos + wu - 1) / wu * wu;  // first take at or past the next chunk
+            }
+            soff[i] = got;
+            counter = pos + wu;
+        }
+        *d.counter = counter;
+    }
 }
 
 // Bitonic sort of up to RB keys ascending into skey[0, P2) (P2 a power of
@@ -1380,7 +1484,7 @@ __device__ __forceinline__ void sort_keys_one_per_thread(const u64 *src, u32 L, 
 // strong overflow) does not fit and nothing was appended.  Moves go to the
 // extra move list, counted in LDS (`local_cnt`) or in st[ST_MOVES].
 template<u32 CAP, int NT = RB>
-__device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, const ResolveLds &lds, u32 *local_cnt) {
+__device__ __forceinline__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, const ResolveLds &lds, u32 *local_cnt) {
     u64 *skey = lds.skey;
     s64 *soff = lds.soff;
     u8 *ssel = lds.ssel;
@@ -1486,9 +1590,13 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
             d.mark[u] = stamp;
             d.oldp[u] = ub;
             if (dst == NO_MOVE) {
+                // the serial step below reads its position and weight from LDS
+                // (vertex weights fit 31 bits: coarse weights are s32)
                 const u32 q = atomicAdd(&ocount, 1u);
                 if (q < OVF_CAP) olist[q] = i;
                 ssel[i] = 2;
+                soff[i] = pos;
+                skey[i] = ((u64) vwt(d, u) << 32) | u;
             } else {
                 d.dest[u] = dst;
             }
@@ -1496,29 +1604,15 @@ __device__ bool resolve_body(const JetDev &d, bool strong, u32 stamp, u32 L, con
         __syncthreads();
         const u32 oc = ocount;
         if (oc > OVF_CAP) return false;
-        if (threadIdx.x == 0 && oc) {
-            // serial overflow in vertex id order (insertion sort of the entries by id)
-            for (u32 a = 1; a < oc; ++a) {
-                const u32 x = olist[a];
-                u32 b = a;
-                while (b > 0 && (u32) skey[olist[b - 1]] > (u32) skey[x]) {
-                    olist[b] = olist[b - 1];
-                    --b;
-                }
-                olist[b] = x;
-            }
-            s64 counter = *d.counter;
-            for (u32 q = 0; q < oc; ++q) {
-                const u32 i = olist[q];
-                const u32 u = (u32) skey[i];
-                const u32 g = (u32) (skey[i] >> 32);
-                const u32 ub = d.work[u];
-                const s64 pos = bucket_prefix(d, g, ub) + soff[i] + d.evict_adjust[ub];
-                d.dest[u] = overflow_dest(es, d.k, pos, vwt(d, u), ub, counter);
-            }
-            *d.counter = counter;
+        if (oc) {
+            if (es_lds) overflow_block<NT>(d, les_of(s_es), lds, oc);
+            else overflow_block<NT>(d, (const s64 *) d.evict_start, lds, oc);
         }
         __syncthreads();
+        for (u32 q = threadIdx.x; q < oc; q += NT) {
+            const u32 i = olist[q], u = (u32) skey[i], id = (u32) soff[i];
+            d.dest[u] = id < d.k ? id : d.work[u];
+        }
     }
     for (u32 i0 = 0; i0 < L; i0 += NT) {
         const u32 i = i0 + threadIdx.x;
@@ -1765,7 +1859,7 @@ __device__ void map_control(const JetDev &d, JetCtl *ctl, const unsigned long lo
 // One step of jet_refine's loop (jet:960-1030) after an iteration: counters,
 // cut and block weight, best-partition tracking, ratio and mode of the next
 // iteration.  Runs on one thread of the control launch.
-__device__ void control_step(const JetDev &d, s64 maxw, const u64 *tot, const JetCtl &in) {
+__device__ __forceinline__ void control_step(const JetDev &d, s64 maxw, const u64 *tot, const JetCtl &in) {
     JetCtl cl = in;  // the control block as loaded by the caller; written back at the end
     JetCtl *ctl = &cl;
     unsigned long long rep[ST_COUNT];
@@ -2402,7 +2496,34 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             s_acc[PT_MISS_DEG] = degs;
         }
         // ---- minibucket prefixes, one wave per block range
-        if (mode) {
+        if (mode && d.width <= WAVE) {
+            // one chunk per block: a wave loads PU blocks' totals before
+            // scanning any, so the loads overlap
+            constexpr u32 NW = BM / WAVE, PU = 8;
+            const u32 i = lane_id();
+            for (u32 p0 = tid / WAVE; p0 < d.k; p0 += PU * NW) {
+                s64 v[PU];
+#pragma unroll
+                for (u32 j = 0; j < PU; ++j) {
+                    const u32 p = p0 + j * NW;
+                    v[j] = p < d.k && i < d.width ? d.btot[(u64) p * d.width + i] : 0;
+                }
+#pragma unroll
+                for (u32 j = 0; j < PU; ++j) {
+                    const u32 p = p0 + j * NW;
+                    if (p >= d.k) break;  // uniform in the wave
+                    const s64 inc = wave_inclusive_scan(v[j]);
+                    const u64 base = (u64) p * d.width;
+                    if (i < d.width) {
+                        d.Q[base + i] = inc - v[j];
+                        d.R[base + i] = inc;
+                        d.btot[base + i] = 0;
+                    }
+                    if (i == WAVE - 1) d.Tseg[p] = inc;  // lanes past the width add 0
+                    if (i == 0 && p == d.k - 1) d.btot[(u64) d.k * d.width] = 0;
+                }
+            }
+        } else if (mode) {
             for (u32 p = tid / WAVE; p < d.k; p += BM / WAVE) {
                 const u64 base = (u64) p * d.width;
                 s64 carry = 0;
@@ -2698,7 +2819,8 @@ __global__ void fb_overflow_kernel(JetDev d, const s64 *roff, const u32 *olist, 
         over_flag[u] = 0;
         const u32 ub = d.work[u];
         const s64 pos = roff[u] + d.evict_adjust[ub];
-        d.dest[u] = overflow_dest(d.evict_start, d.k, pos, vwt(d, u), ub, counter);
+        const u32 id = overflow_chunk(d.evict_start, d.k, pos, vwt(d, u), counter);
+        d.dest[u] = id < d.k ? id : ub;
     }
     *d.counter = counter;
 }
@@ -2743,7 +2865,7 @@ struct JetHost {
     const DGraph &g;
     const GraphMeta &meta;
     JetDev d{};
-    u32 G = 16, Gs = 16, sweep_threads = JB;
+    u32 G = 16, Gs = 16, Gb = 8, sweep_threads = JB;
     bool big_resolve = false;
     bool block_mode = false;
     u32 bm_groups = 1;
@@ -2950,7 +3072,8 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     // single-workgroup form for small levels without hubs
     {
         const size_t cgb = conn_group_bytes(k);
-        h.bm_groups = (u32) std::max<size_t>(1, std::min<size_t>(BM / meta.G, BM_CONN_LDS / cgb));
+        h.Gb = cap_group(meta.G, c.block_group_cap);
+        h.bm_groups = (u32) std::max<size_t>(1, std::min<size_t>(BM / h.Gb, BM_CONN_LDS / cgb));
         h.bm_lds = std::max(bm_sweep_lds_bytes(k, h.bm_groups), resolve_lds_bytes(BM_RCAP));
         // one workgroup walks every edge each iteration: dense small levels
         // (coarse levels of skewed graphs) stay on the multi-launch form
@@ -2960,8 +3083,8 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
             // a launch's LDS is its dynamic size plus the kernel's own __shared__ variables
             h.bm_state_off = (u32) ((h.bm_lds + 15) & ~(size_t) 15);
             h.bm_ls = c.block_lds &&
-                      h.bm_state_off + bm_state_bytes(n, k) + jet_block_static_lds(meta.G, true) <= c.max_lds;
-            require(h.bm_ls || h.bm_lds + jet_block_static_lds(meta.G, false) <= c.max_lds,
+                      h.bm_state_off + bm_state_bytes(n, k) + jet_block_static_lds(h.Gb, true) <= c.max_lds;
+            require(h.bm_ls || h.bm_lds + jet_block_static_lds(h.Gb, false) <= c.max_lds,
                     "single-workgroup Jet does not fit the LDS");
         }
     }
@@ -2984,7 +3107,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     // lane-group geometry: G lanes per vertex; the sweep keeps one LDS
     // k-counter array per group, so wide k shrinks the groups per workgroup and,
     // if one wave's groups do not fit, widens the groups
-    h.G = meta.G;
+    h.G = cap_group(meta.G, c.group_cap);
     h.flat = meta.max_deg > 8u * meta.G;  // skewed: a lane group per vertex would wait on the heaviest ones
     {
         const size_t cgb = conn_group_bytes(k);
@@ -3088,7 +3211,7 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
     const u32 bm_iters = tracing ? TRACE_RING / 2 : (1u << 20);
     auto submit = [&](int sidx) {
         if (h.block_mode) {
-            HEIPA_DISPATCH_G(h.G, h.template launch_block, bm_iters);
+            HEIPA_DISPATCH_G(h.Gb, h.template launch_block, bm_iters);
         } else if (gx.e) {
             HIP_CHECK(hipGraphLaunch(gx.e, c.stream));
         } else {

@@ -36,6 +36,8 @@ struct heipa_pool {
     std::deque<Queued> queue;
     uint64_t inflight = 0;
     bool stop = false;
+    // the launch-shape options heipa_pool_autotune searches, as last set
+    uint32_t group_cap = 64, block_group_cap = 8, block_jet = 2048;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -256,6 +258,57 @@ int heipa_pool_set_option(heipa_pool *p, int option, uint64_t value) {
         const int rc = heipa_ctx_set_option(c, option, value);
         if (rc != HEIPA_OK) return rc;
     }
+    const uint32_t v = (uint32_t) std::min<uint64_t>(value, 0xFFFFFFFFull);
+    if (option == HEIPA_OPT_GROUP_CAP) p->group_cap = v;
+    else if (option == HEIPA_OPT_BLOCK_GROUP_CAP) p->block_group_cap = v;
+    else if (option == HEIPA_OPT_BLOCK_JET) p->block_jet = v;
+    return HEIPA_OK;
+}
+
+int heipa_pool_autotune(heipa_pool *p, heipa_job *sample, uint32_t count, heipa_tune *out) {
+    if (!p || !sample || count == 0 || !out) return HEIPA_ERR_INVALID;
+    *out = heipa_tune{};
+    int status = HEIPA_OK;
+    auto run = [&]() -> double {
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = heipa_pool_run(p, sample, count);
+        for (uint32_t i = 0; rc == HEIPA_OK && i < count; ++i) rc = sample[i].status;
+        if (rc != HEIPA_OK && status == HEIPA_OK) status = rc;
+        out->runs++;
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    struct Knob {
+        int option;
+        uint32_t cand[3];
+        uint32_t *cur;
+    };
+    Knob knobs[3] = {{HEIPA_OPT_GROUP_CAP, {64, 16, 8}, &p->group_cap},
+                     {HEIPA_OPT_BLOCK_GROUP_CAP, {16, 8, 4}, &p->block_group_cap},
+                     {HEIPA_OPT_BLOCK_JET, {1024, 2048, 4096}, &p->block_jet}};
+    run();  // warm-up: first-touch costs do not belong to any setting
+    double best = run();
+    out->ms_start = best;
+    for (Knob &kb: knobs) {
+        uint32_t chosen = *kb.cur;
+        for (uint32_t v: kb.cand) {
+            if (status != HEIPA_OK) break;
+            if (v == chosen) continue;
+            const int rc = heipa_pool_set_option(p, kb.option, v);
+            if (rc != HEIPA_OK) return rc;
+            const double t = run();
+            if (t < 0.98 * best) {
+                best = t;
+                chosen = v;
+            }
+        }
+        const int rc = heipa_pool_set_option(p, kb.option, chosen);
+        if (rc != HEIPA_OK) return rc;
+        if (status != HEIPA_OK) return status;
+    }
+    out->group_cap = p->group_cap;
+    out->block_group_cap = p->block_group_cap;
+    out->block_jet = p->block_jet;
+    out->ms_best = best;
     return HEIPA_OK;
 }
 

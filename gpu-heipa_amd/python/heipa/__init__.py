@@ -31,6 +31,8 @@ OPT_HM_WORKER_BYTES = 8
 OPT_RACE = 9
 OPT_HIER_CACHE = 10
 OPT_GRAPH_KEY = 11
+OPT_GROUP_CAP = 12
+OPT_BLOCK_GROUP_CAP = 13
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -77,6 +79,11 @@ class Job(ctypes.Structure):
                 ("maxw", ctypes.c_int64), ("estimate", ctypes.c_uint64), ("t_submit", ctypes.c_double),
                 ("t_start", ctypes.c_double), ("t_done", ctypes.c_double), ("state", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("peak", ctypes.c_uint64), ("graph_key", ctypes.c_uint64)]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("group_cap", ctypes.c_uint32), ("block_group_cap", ctypes.c_uint32), ("block_jet", ctypes.c_uint32),
+                ("runs", ctypes.c_uint32), ("ms_start", ctypes.c_double), ("ms_best", ctypes.c_double)]
 
 
 class LevelTrace(ctypes.Structure):
@@ -153,6 +160,7 @@ def lib():
             "heipa_pool_wait": (c_int, [vp, ctypes.POINTER(Job), c_dbl]),
             "heipa_pool_clock": (c_dbl, [vp]),
             "heipa_pool_set_option": (c_int, [vp, c_int, c_u64]),
+            "heipa_pool_autotune": (c_int, [vp, ctypes.POINTER(Job), c_u32, ctypes.POINTER(Tune)]),
             "heipa_prof_enable": (c_int, [vp, c_int]),
             "heipa_prof_reset": (c_int, [vp]),
             "heipa_prof_filter": (c_int, [vp, ctypes.c_char_p]),
@@ -616,6 +624,21 @@ class Pool:
         for i, j in enumerate(jobs):  # copy the results back into the caller's objects
             ctypes.memmove(ctypes.addressof(j), ctypes.addressof(arr[i]), ctypes.sizeof(Job))
         return jobs
+
+    def autotune(self, sample):
+        """heipa_pool_autotune over the sample jobs: the launch-shape setting it
+        leaves on the workers, as a dict (the sample jobs hold their last run's
+        results)."""
+        _torch().cuda.synchronize()
+        arr = (Job * len(sample))(*sample)
+        t = Tune()
+        rc = self.L.heipa_pool_autotune(self.h, arr, len(sample), ctypes.byref(t))
+        if rc != OK:
+            raise HeipaError("heipa_pool_autotune", rc)
+        for i, j in enumerate(sample):
+            ctypes.memmove(ctypes.addressof(j), ctypes.addressof(arr[i]), ctypes.sizeof(Job))
+        return {"group_cap": t.group_cap, "block_group_cap": t.block_group_cap, "block_jet": t.block_jet,
+                "runs": t.runs, "ms_start": round(t.ms_start, 3), "ms_best": round(t.ms_best, 3)}
 
     def submit(self, job):
         """Served form: queue one job (it must stay alive until done)."""

@@ -233,6 +233,19 @@ double heipa_pool_clock(const heipa_pool *pool);
 /* heipa_ctx_set_option on every worker's context (call while no job runs); HEIPA_OPT_HIER_CACHE makes
  * one cache that all workers share */
 int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
+/* Autotuner of the batched form (no counterpart in the reference: its launch shapes are fixed).  Runs
+ * the sample jobs through the pool once per candidate setting of three options that change launch
+ * shapes but never results -- HEIPA_OPT_GROUP_CAP in {64, 16, 8}, HEIPA_OPT_BLOCK_GROUP_CAP in
+ * {16, 8, 4}, HEIPA_OPT_BLOCK_JET in {1024, 2048, 4096} -- one option at a time starting from the
+ * current setting, takes a candidate when the sample runs at least 2% faster with it, and leaves the
+ * chosen setting on every worker (call while no job runs).  The sample's outputs are those of its
+ * last run; a failed sample job ends the tuning with its status. */
+typedef struct {
+    uint32_t group_cap, block_group_cap, block_jet;  /* the chosen setting */
+    uint32_t runs;                                   /* sample runs, the untimed first one included */
+    double ms_start, ms_best;                        /* sample wall time at the starting and the chosen setting */
+} heipa_tune;
+int heipa_pool_autotune(heipa_pool *pool, heipa_job *sample, uint32_t count, heipa_tune *out);
 /* heipa_hier_cache_stats of the pool's shared cache */
 int heipa_pool_hier_cache_stats(const heipa_pool *pool, uint64_t *entries, uint64_t *bytes, uint64_t *levels_reused,
                                 uint64_t *levels_built);
@@ -290,6 +303,13 @@ int heipa_pool_hier_cache_stats(const heipa_pool *pool, uint64_t *entries, uint6
  * (0, the default: no reuse).  A key that comes back with another n, m or total weight replaces its
  * entry; the caller keeps one key per graph. */
 #define HEIPA_OPT_GRAPH_KEY 11
+/* HEIPA_OPT_GROUP_CAP: the widest lane group (lanes per vertex) of the multi-launch Jet's afterburner,
+ * apply and row launches (default 64, i.e. the power of two at or above the level's average degree,
+ * at most 64; a smaller value rounds down to a power of two, at least 4).  Results do not depend on it.
+ * HEIPA_OPT_BLOCK_GROUP_CAP: the same for the single-workgroup Jet (default 8: more vertices in flight
+ * in the one workgroup).  heipa_pool_autotune picks both for a pool. */
+#define HEIPA_OPT_GROUP_CAP 12
+#define HEIPA_OPT_BLOCK_GROUP_CAP 13
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 /* dst uses src's hierarchy cache (HEIPA_OPT_HIER_CACHE) from now on: one cache for several contexts */
 int heipa_ctx_share_hier_cache(heipa_ctx *dst, const heipa_ctx *src);

@@ -213,6 +213,25 @@ def test_jet_block_state(ctx, name, lds, cap, ultra, level, skew, k):
         ctx.set_option(heipa.OPT_RESOLVE_CAP, 1 << 32)
 
 
+@pytest.mark.parametrize("opts", [{heipa.OPT_BLOCK_JET: 0, heipa.OPT_GROUP_CAP: 4},
+                                  {heipa.OPT_BLOCK_JET: 1 << 20, heipa.OPT_BLOCK_GROUP_CAP: 64},
+                                  {heipa.OPT_BLOCK_JET: 1 << 20, heipa.OPT_BLOCK_GROUP_CAP: 4}],
+                         ids=["multi-cap4", "block-cap64", "block-cap4"])
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (True, 0, 0.3, 64)])
+def test_jet_group_caps(ctx, graph, opts, ultra, level, skew, k):
+    """Lane groups narrower than the level's average degree (afterburner,
+    apply and rows of the multi-launch form; the single-workgroup form) or as
+    wide as it give the same iterations."""
+    for o, v in opts.items():
+        ctx.set_option(o, v)
+    try:
+        test_jet(ctx, graph, ultra, level, skew, k)
+    finally:
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
+        ctx.set_option(heipa.OPT_GROUP_CAP, 64)
+        ctx.set_option(heipa.OPT_BLOCK_GROUP_CAP, 8)
+
+
 @pytest.mark.parametrize("copies", [2, 32])
 @pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (False, 1, 0.7, 7)])
 def test_jet_bucket_copies(ctx, graph, copies, ultra, level, skew, k):

@@ -8,6 +8,8 @@ import pytest
 
 import heipa
 from graphs import star_plus_grid
+from heipa import workloads
+from test_gpu_parity import _dev
 
 pytestmark = pytest.mark.gpu
 
@@ -129,3 +131,29 @@ def test_pool_failed_job_does_not_affect_the_others(ctx):
             assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
     finally:
         pool.close()
+
+
+def test_pool_autotune_keeps_results(ctx):
+    """The autotuner times a sample under each candidate launch shape and
+    leaves the fastest on the workers; the sample's partitions (from its last
+    run) and a later run under the chosen setting equal the alone runs."""
+    reqs = workloads.batched(12)
+    dgs = [_dev(g) for g, _, _ in reqs]
+    want = [heipa.to_numpy_u32(ctx.partition(dg, spec["k"], 0.03, 1, False)[0]).copy()
+            for dg, (_, spec, _) in zip(dgs, reqs)]
+    pool = heipa.Pool(4, 256 << 20)
+    try:
+        sample = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **spec) for dg, (_, spec, _) in zip(dgs, reqs)]
+        t = pool.autotune(sample)
+        assert t["group_cap"] in (64, 16, 8) and t["block_group_cap"] in (16, 8, 4)
+        assert t["block_jet"] in (1024, 2048, 4096)
+        assert t["runs"] == 8 and 0 < t["ms_best"] <= t["ms_start"]
+        for j, w, dg in zip(sample, want, dgs):
+            assert j.status == heipa.OK and np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], w)
+        jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **spec) for dg, (_, spec, _) in zip(dgs, reqs)]
+        pool.run(jobs)
+        for j, w, dg in zip(jobs, want, dgs):
+            assert j.status == heipa.OK and np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], w)
+    finally:
+        pool.close()
+

@@ -5,7 +5,10 @@ from run to run, so the bar is statistical: over seeds 1..10 the cut (or J)
 is at most 2% above the deterministic mode's in geometric mean (a lower one
 passes: single seeds scatter by several percent either way on these graphs,
 and racing mode has come out up to 3% lower), and every partition meets lmax
-wherever the deterministic one does."""
+wherever the deterministic one does.  Each seed's racing side is the
+geometric mean of RUNS racing runs: one racing draw per seed scatters by up
+to 14% on the 512x512 grid, which puts the standard error of a 10-draw mean
+at the 2% bar itself (one such draw came out at 1.0207)."""
 import math
 
 import numpy as np
@@ -18,6 +21,7 @@ from test_gpu_parity import _dev
 pytestmark = pytest.mark.gpu
 
 SEEDS = range(1, 11)
+RUNS = 3
 
 
 def _geomean(xs):
@@ -36,9 +40,12 @@ def _race_vs_det(ctx, solve):
             ctx.set_option(heipa.OPT_RACE, 0)
             qd, okd = solve(s)
             ctx.set_option(heipa.OPT_RACE, 1)
-            qr, okr = solve(s)
-            assert okr or not okd, (s, qd, qr)
-            ratios.append(qr / qd)
+            qs = []
+            for _ in range(RUNS):
+                qr, okr = solve(s)
+                assert okr or not okd, (s, qd, qr)
+                qs.append(qr)
+            ratios.append(_geomean(qs) / qd)
     finally:
         ctx.set_option(heipa.OPT_RACE, 0)
     gm = _geomean(ratios)
