@@ -446,10 +446,7 @@ def run_headline(args):
     pool.close()
 
     ctx = heipa.Context(0, arena)
-    if tune:
-        ctx.set_option(heipa.OPT_GROUP_CAP, tune["group_cap"])
-        ctx.set_option(heipa.OPT_BLOCK_GROUP_CAP, tune["block_group_cap"])
-        ctx.set_option(heipa.OPT_BLOCK_JET, tune["block_jet"])
+    _apply_tune(ctx, tune)
     for i in sample[:2]:
         ctx.partition(dgs[i], reqs[i][1]["k"], 0.03, 1, False)
     torch.cuda.synchronize()
@@ -464,7 +461,7 @@ def run_headline(args):
     plain_rate = es / plain_s
     same_graph_batched = es / sample_batched_s
 
-    served = served_sample(args)
+    served = served_sample(args, tune)
     res1, _, _, ctx1 = measure_single(args, "rgg20-ultra-k64", 5, 2)
     ctx1.close()
     cpu = None if args.no_cpu_baseline else cpu_batched(args, reqs, edges)
@@ -553,10 +550,21 @@ def cpu_batched(args, reqs, edges):
                       f"threads, {dt:.1f} s"}
 
 
-def served_sample(args):
+def _apply_tune(obj, tune):
+    """the autotuned launch shapes on a context or a pool (results do not change)"""
+    import heipa
+    if tune:
+        obj.set_option(heipa.OPT_GROUP_CAP, tune["group_cap"])
+        obj.set_option(heipa.OPT_BLOCK_GROUP_CAP, tune["block_group_cap"])
+        obj.set_option(heipa.OPT_BLOCK_JET, tune["block_jet"])
+
+
+def served_sample(args, tune=None):
     """Poisson arrivals of the served request mix at 0.8x the plain path's
     request capacity (measured on the first 24 requests): latency
-    percentiles, edges/s over the makespan, saturated capacity."""
+    percentiles, edges/s over the makespan, saturated capacity.  `tune`: the
+    launch shapes the batched leg's autotuner chose, set on the pool and on
+    the plain context alike."""
     import numpy as np
     import torch
     import heipa
@@ -578,6 +586,7 @@ def served_sample(args):
     pool0.close()
     del probe
     ctx = heipa.Context(0, arena)
+    _apply_tune(ctx, tune)
     sample = list(range(min(24, len(reqs))))
     for i in sample[:2]:
         _run_alone(ctx, dgs[i], reqs[i][1])
@@ -590,6 +599,7 @@ def served_sample(args):
     capacity = len(sample) / plain_s
     lam = 0.8 * capacity
     pool = heipa.Pool(args.slots, arena)
+    _apply_tune(pool, tune)
     pool.run([heipa.Pool.job(dgs[i], imbalance=0.03, seed=1, **reqs[i][1]) for i in range(min(2 * args.slots, len(reqs)))])
     if args.served_repeat:
         pool.set_option(heipa.OPT_HIER_CACHE, 64 << 30)
@@ -632,6 +642,8 @@ def served_sample(args):
                 "distinct_graphs": args.served_repeat, "graphs_held": cache[0], "bytes": cache[1],
                 "levels_reused": cache[2], "levels_built": cache[3]},
             "config": {"requests": len(reqs), "n_min": 1e4, "n_max": args.served_nmax,
+                       "launch_shapes": ({key: tune[key] for key in ("group_cap", "block_group_cap", "block_jet")}
+                                         if tune else "defaults"),
                        "offered_rate_per_s": round(lam, 3), "plain_capacity_per_s": round(capacity, 3),
                        "undirected_edges": sum(edges), "slots": args.slots,
                        "partition_requests": sum(1 for r in reqs if "k" in r[1]),
