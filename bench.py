@@ -81,7 +81,10 @@ def main():
     ap.add_argument("--profile-all", action="store_true", help="print the per-kernel event profile to stderr")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device (the known-size kernel that scales HBM counters)")
-    ap.add_argument("--slots", type=int, default=8, help="batched/served: pool workers")
+    ap.add_argument("--slots", type=int, default=8, help="batched: pool workers (and served, alone)")
+    ap.add_argument("--served-slots", type=int, default=12,
+                    help="headline: the served leg's pool workers (measured best for the served mix: 21.2M "
+                         "edges/s at 12 against 20.1M at 8, 20.7M at 16, 16.5M at 6)")
     ap.add_argument("--hm-workers", type=int, default=None,
                     help="HM: worker contexts for the subtrees below the first split (default: the library's)")
     ap.add_argument("--block-jet-n", type=int, default=None,
@@ -461,7 +464,7 @@ def run_headline(args):
     plain_rate = es / plain_s
     same_graph_batched = es / sample_batched_s
 
-    served = served_sample(args, tune)
+    served = served_sample(args, tune, args.served_slots)
     res1, _, _, ctx1 = measure_single(args, "rgg20-ultra-k64", 5, 2)
     ctx1.close()
     cpu = None if args.no_cpu_baseline else cpu_batched(args, reqs, edges)
@@ -559,7 +562,7 @@ def _apply_tune(obj, tune):
         obj.set_option(heipa.OPT_BLOCK_JET, tune["block_jet"])
 
 
-def served_sample(args, tune=None):
+def served_sample(args, tune=None, slots=None):
     """Poisson arrivals of the served request mix at 0.8x the plain path's
     request capacity (measured on the first 24 requests): latency
     percentiles, edges/s over the makespan, saturated capacity.  `tune`: the
@@ -598,9 +601,10 @@ def served_sample(args, tune=None):
     plain_rate = sum(edges[i] for i in sample) / plain_s
     capacity = len(sample) / plain_s
     lam = 0.8 * capacity
-    pool = heipa.Pool(args.slots, arena)
+    slots = slots or args.slots
+    pool = heipa.Pool(slots, arena)
     _apply_tune(pool, tune)
-    pool.run([heipa.Pool.job(dgs[i], imbalance=0.03, seed=1, **reqs[i][1]) for i in range(min(2 * args.slots, len(reqs)))])
+    pool.run([heipa.Pool.job(dgs[i], imbalance=0.03, seed=1, **reqs[i][1]) for i in range(min(2 * slots, len(reqs)))])
     if args.served_repeat:
         pool.set_option(heipa.OPT_HIER_CACHE, 64 << 30)
     jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, graph_key=key, **spec)
@@ -645,7 +649,7 @@ def served_sample(args, tune=None):
                        "launch_shapes": ({key: tune[key] for key in ("group_cap", "block_group_cap", "block_jet")}
                                          if tune else "defaults"),
                        "offered_rate_per_s": round(lam, 3), "plain_capacity_per_s": round(capacity, 3),
-                       "undirected_edges": sum(edges), "slots": args.slots,
+                       "undirected_edges": sum(edges), "slots": slots,
                        "partition_requests": sum(1 for r in reqs if "k" in r[1]),
                        "mapping_requests": sum(1 for r in reqs if "hier" in r[1]),
                        "mix": "partition k uniform in 8..256 (70%), HM 4:8:6 or 2:4:8 (30%); grid / RGG "
