@@ -182,10 +182,13 @@ struct JetDev {
     s64 *counter;       // strong-rebalance overflow position
     const s32 *D;       // k x k distances: mapping objective J (ProMap); null: edge cut
     // BlockConn rows of the wide vertices (degree >= k); rflag null when the level has none
-    u8 *rflag;          // n: bit 0 wide, bit 1 a neighbour is in this iteration's move list
+    u8 *rflag;          // n: bit 0 wide, bit 1 a neighbour is in this iteration's move list,
+                        // bit 2 its neighbour counts are kept (wcnt)
     u32 *wid;           // n: rank among the wide vertices
     u32 *wsz;           // per wide vertex: the row's size (BlockConn sizes)
     u32 *wV, *wP;       // per wide vertex, W words each: blocks of its valid entries / of positive ones
+    u32 *wcnt;          // per wide vertex, k counts of its neighbours per block, kept by the moves for
+                        // the counted ones: k <= 64 and ROW_THREAD_DEG < degree <= hub_deg
     u32 W;              // words per block mask, ceil(k / 32)
     u32 race;           // atomic order: minibucket offsets and strong overflow positions from fetch-adds
 };
@@ -229,10 +232,34 @@ __device__ __forceinline__ s64 max_block_weight(const JetDev &d, bool strong) {
     return a < d.lmax - 100 ? d.lmax - 100 : a;
 }
 
-// a neighbour of a vertex in the move list: its row is updated after the
-// iteration (benign race: every writer stores the same value)
-__device__ __forceinline__ void row_touch(const JetDev &d, u32 v) {
-    if (d.rflag && (d.rflag[v] & 1u)) d.rflag[v] = 3;
+constexpr u32 ROW_THREAD_DEG = 32;  // wide vertices up to this degree walk their adjacency in one thread
+
+// a neighbour v of a vertex moving from block ou to nu: a wide v's row is
+// updated after the iteration (benign race on the flag: every writer stores
+// the same value).  A counted v's neighbour counts follow the move, as the
+// reference's update_small follows each moved vertex's edges
+// (block_conn.h:208-345): its row update then reads k counts instead of its
+// adjacency.  Vertices of low degree are cheaper to walk, and a hub's counts
+// would serialise the moves' atomics on k addresses, so neither is counted.
+__device__ __forceinline__ void row_touch(const JetDev &d, u32 v, u32 ou, u32 nu) {
+    const u8 f = d.rflag ? d.rflag[v] : (u8) 0;
+    if (f & 1u) {
+        if (!(f & 2u)) d.rflag[v] = f | 2u;  // most neighbours of a move list are touched many times
+        if ((f & 4u) && ou != nu) {
+            const u64 base = (u64) d.wid[v] * d.k;
+            atomicSub(&d.wcnt[base + ou], 1u);
+            atomicAdd(&d.wcnt[base + nu], 1u);
+        }
+    }
+}
+
+// the block mask of a counted vertex's positive entries from its counts
+__device__ __forceinline__ u64 row_count_mask(const JetDev &d, u32 v) {
+    const u64 base = (u64) d.wid[v] * d.k;
+    u64 mask = 0;
+    for (u32 b = 0; b < d.k; ++b)
+        if (__hip_atomic_load(&d.wcnt[base + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) mask |= 1ull << b;
+    return mask;
 }
 
 // the wide vertex u's row holds k entries: its valid entries include the
@@ -1726,7 +1753,7 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
                 delta += mv ? x : 2 * x;
             }
             stale_of(d)[v[q]] = 3;
-            row_touch(d, v[q]);
+            row_touch(d, v[q], ou, nu);
         }
     }
     return delta;
@@ -2127,7 +2154,7 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
                     acc += mv ? x : 2 * x;
                 }
                 stale_of(d)[v[q]] = 3;
-                row_touch(d, v[q]);
+                row_touch(d, v[q], ou, nu);
             }
         }
         if (t < cnt && !hub) {
@@ -2216,26 +2243,60 @@ __device__ __forceinline__ void group_sync() {
     else __syncthreads();
 }
 
-// one wide vertex u per group of G lanes; wm: the group's W LDS words
+// one wide vertex u per group of G lanes; wm: the group's W LDS words.  The
+// group walks u's adjacency (at init counting a counted vertex's neighbours
+// per block in cl, the group's k LDS counters), or, after init, reads a
+// counted vertex's counts.  A walk that needs no counts ends once every
+// block has been seen (P is then all k blocks): a hub of an HM split
+// (k <= 8) sees them all in its first rounds.
 template<int G>
 __device__ __forceinline__ void row_update(const JetDev &d, u32 *wm, u32 u, u32 lane, u32 rmode, bool active,
-                                           u32 *red) {
-    if (active)
-        for (u32 i = lane; i < d.W; i += G) wm[i] = 0;
-    group_sync<G>();
-    if (active) {
-        const u64 e = d.xadj[u + 1];
-        for (u64 j0 = (u64) d.xadj[u] + lane; j0 < e; j0 += (u64) G * ADJ_UNROLL) {
-            u32 v[ADJ_UNROLL];
-#pragma unroll
-            for (u32 q = 0; q < ADJ_UNROLL; ++q) v[q] = j0 + (u64) q * G < e ? d.adj[j0 + (u64) q * G] : 0u;
-#pragma unroll
-            for (u32 q = 0; q < ADJ_UNROLL; ++q)
-                if (j0 + (u64) q * G < e) {
-                    const u32 b = d.work[v[q]];
-                    atomicOr(&wm[b >> 5], 1u << (b & 31));
-                }
+                                           u32 *red, u32 *cl = nullptr) {
+    const bool counted = active && (d.rflag[u] & 4u);
+    // the same branch for every group of a launch: at init all walk, after
+    // init the counted vertices' groups read counts (a hub's workgroup and,
+    // for k > 64, every group walk: none of them is counted)
+    if (rmode == ROW_INIT || !counted) {
+        if (active) {
+            for (u32 i = lane; i < d.W; i += G) wm[i] = 0;
+            if (counted)
+                for (u32 b = lane; b < d.k; b += G) cl[b] = 0;
         }
+        group_sync<G>();
+        const bool can_stop = d.W <= 2 && !counted;
+        if (active) {
+            const u64 e = d.xadj[u + 1];
+            // the loop runs on the group's base, so every lane of the group
+            // runs the same rounds and meets the same barriers
+            for (u64 j00 = d.xadj[u]; j00 < e; j00 += (u64) G * ADJ_UNROLL) {
+                const u64 j0 = j00 + lane;
+                u32 v[ADJ_UNROLL];
+#pragma unroll
+                for (u32 q = 0; q < ADJ_UNROLL; ++q) v[q] = j0 + (u64) q * G < e ? d.adj[j0 + (u64) q * G] : 0u;
+#pragma unroll
+                for (u32 q = 0; q < ADJ_UNROLL; ++q)
+                    if (j0 + (u64) q * G < e) {
+                        const u32 b = d.work[v[q]];
+                        atomicOr(&wm[b >> 5], 1u << (b & 31));
+                        if (counted) atomicAdd(&cl[b], 1u);
+                    }
+                if (can_stop) {
+                    group_sync<G>();  // the group's bits of this round
+                    const u32 seen = (u32) __popc(wm[0]) + (d.W > 1 ? (u32) __popc(wm[1]) : 0u);
+                    group_sync<G>();  // every lane has read them before the next round adds any
+                    if (seen == d.k) break;
+                }
+            }
+        }
+        group_sync<G>();
+        if (counted) {
+            const u64 base = (u64) d.wid[u] * d.k;
+            for (u32 b = lane; b < d.k; b += G) d.wcnt[base + b] = cl[b];
+        }
+    } else if (lane == 0) {
+        const u64 mask = row_count_mask(d, u);  // W <= 2
+        wm[0] = (u32) mask;
+        if (d.W > 1) wm[1] = (u32) (mask >> 32);
     }
     group_sync<G>();
     u32 pc = 0;
@@ -2263,10 +2324,66 @@ __device__ __forceinline__ void row_update(const JetDev &d, u32 *wm, u32 u, u32 
                 ns = sz > pc ? sz : pc;
             }
             d.wsz[r] = ns;
-            d.rflag[u] = 1;
+            d.rflag[u] &= (u8) ~2u;
         }
     }
     group_sync<G>();
+}
+
+// Wide vertices of k <= 64 that are not hubs, one thread each, the block
+// mask in a register: those of degree <= ROW_THREAD_DEG walk their adjacency,
+// the counted ones read their counts (at init the row kernel counts them).
+// On skewed graphs with small k most vertices are wide and a move list
+// touches most of them; a lane group per vertex spent its time in the
+// group's fixed costs, one vertex after another.
+__global__ __launch_bounds__(JB) void row_thread_kernel(JetDev d, u32 init) {
+    u32 rmode = ROW_INIT;
+    if (!init) {
+        const CtlHead hd = ctl_head(d.ctl);
+        if (!live(hd)) return;
+        rmode = d.ctl->last_moves > d.n / 10 ? ROW_LARGE : ROW_SMALL;
+    }
+    for (u64 v = (u64) blockIdx.x * JB + threadIdx.x; v < d.n; v += (u64) gridDim.x * JB) {
+        const u8 f = d.rflag[v];
+        if ((f & 3u) != 3u) continue;
+        u64 mask = 0;
+        if (f & 4u) {
+            if (init) continue;  // the row kernel counts them
+            mask = row_count_mask(d, (u32) v);
+        } else {
+            const u32 xb = d.xadj[v], xe = d.xadj[v + 1];
+            if (xe - xb > ROW_THREAD_DEG) continue;  // a hub
+            for (u32 j = xb; j < xe; j += 4) {
+                u32 nb[4];
+#pragma unroll
+                for (u32 q = 0; q < 4; ++q) nb[q] = j + q < xe ? d.adj[j + q] : 0u;
+#pragma unroll
+                for (u32 q = 0; q < 4; ++q)
+                    if (j + q < xe) mask |= 1ull << d.work[nb[q]];
+            }
+        }
+        const u32 pc = (u32) __popcll(mask);
+        const u32 r = d.wid[v];
+        const u32 sz = d.wsz[r];
+        const bool keep = rmode == ROW_SMALL && sz == d.k;
+        for (u32 i = 0; i < d.W; ++i) {
+            const u64 o = (u64) r * d.W + i;
+            const u32 P = (u32) (mask >> (32 * i));
+            d.wV[o] = keep ? (d.wV[o] | P) : P;
+            d.wP[o] = P;
+        }
+        u32 ns;
+        if (rmode == ROW_INIT) {
+            ns = pc;
+        } else if (rmode == ROW_LARGE) {
+            ns = pc + (pc / 4 < 3 ? 3 : pc / 4);
+            if (ns >= d.k) ns = d.k;
+        } else {
+            ns = sz > pc ? sz : pc;
+        }
+        d.wsz[r] = ns;
+        d.rflag[v] = f & (u8) ~2u;
+    }
 }
 
 // the flagged wide vertices that are not hubs, G lanes each; init: every
@@ -2284,6 +2401,8 @@ __global__ __launch_bounds__(JB) void row_kernel(JetDev d, u32 init) {
     const u32 groups = JB / G, lane = threadIdx.x % G, gi = threadIdx.x / G;
     u32 *list = rsm;
     u32 *wm = rsm + SWEEP_PIECE + gi * d.W;
+    // the counted vertices' counts at init: k LDS counters per group (k <= 64)
+    u32 *cl = init && d.W <= 2 ? rsm + SWEEP_PIECE + groups * d.W + gi * d.k : nullptr;
     for (u64 p0 = (u64) blockIdx.x * SWEEP_PIECE; p0 < d.n; p0 += (u64) gridDim.x * SWEEP_PIECE) {
         const u64 p1 = p0 + SWEEP_PIECE < d.n ? p0 + SWEEP_PIECE : d.n;
         if (threadIdx.x == 0) s_nlist = 0;
@@ -2291,14 +2410,18 @@ __global__ __launch_bounds__(JB) void row_kernel(JetDev d, u32 init) {
         for (u64 v0 = p0; v0 < p1; v0 += JB) {
             const u64 vv = v0 + threadIdx.x;
             bool take = false;
-            if (vv < p1 && d.rflag[vv] == 3) take = degree(d, (u32) vv) <= d.hub_deg;
+            // k <= 64: the thread kernel takes the rest (the counted ones after init)
+            if (vv < p1 && (d.rflag[vv] & 3u) == 3u) {
+                const u32 dg = degree(d, (u32) vv);
+                take = dg <= d.hub_deg && (d.W > 2 || (init && dg > ROW_THREAD_DEG));
+            }
             append_local(take, (u32) vv, list, &s_nlist);
         }
         __syncthreads();
         const u32 nl = s_nlist;
         for (u32 r0 = 0; r0 < nl; r0 += groups) {
             const u32 idx = r0 + gi;
-            row_update<G>(d, wm, idx < nl ? list[idx] : 0u, lane, rmode, idx < nl, nullptr);
+            row_update<G>(d, wm, idx < nl ? list[idx] : 0u, lane, rmode, idx < nl, nullptr, cl);
         }
         __syncthreads();
     }
@@ -2315,7 +2438,7 @@ __global__ __launch_bounds__(JB) void row_hub_kernel(JetDev d, const u32 *hubs, 
         rmode = d.ctl->last_moves > d.n / 10 ? ROW_LARGE : ROW_SMALL;
     }
     const u32 u = hubs[blockIdx.x];
-    if (d.rflag[u] != 3) return;
+    if ((d.rflag[u] & 3u) != 3u) return;
     row_update<JB>(d, rsm, u, threadIdx.x, rmode, true, red);
 }
 
@@ -2650,7 +2773,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
                 __syncthreads();
                 for (u32 v0 = p0; v0 < p1; v0 += BM) {
                     const u32 v = v0 + tid;
-                    append_local(v < p1 && d.rflag[v] == 3, v, list, &s_nlist);
+                    append_local(v < p1 && (d.rflag[v] & 3u) == 3u, v, list, &s_nlist);
                 }
                 __syncthreads();
                 const u32 nl = s_nlist;
@@ -2852,10 +2975,13 @@ struct WideDst {
     u32 k;
     u32 *wid;
     u8 *rflag;
+    u32 hub_deg;
     __device__ void operator()(u64 u, u32 pos) const {
-        const bool w = xadj[u + 1] - xadj[u] >= k;
+        const u32 deg = xadj[u + 1] - xadj[u];
+        const bool w = deg >= k;
+        const bool counted = w && k <= 64 && deg > ROW_THREAD_DEG && deg <= hub_deg;
         wid[u] = w ? pos : ~0u;
-        rflag[u] = w ? 3 : 0;  // the init pass builds every wide row
+        rflag[u] = w ? (counted ? 7 : 3) : 0;  // the init pass builds every wide row
     }
 };
 
@@ -2884,17 +3010,19 @@ struct JetHost {
 
     template<int GG>
     void launch_rows(u32 init) {
-        const size_t smem = 4ull * SWEEP_PIECE + 4ull * (JB / GG) * d.W;
-        launch(c, "jet_rows", 0.0, dim3(grid_for(c, div_up(d.n, SWEEP_PIECE), 1, 1)), dim3(JB), smem,
+        const size_t smem = 4ull * SWEEP_PIECE + 4ull * (JB / GG) * d.W + (init && d.W <= 2 ? 4ull * (JB / GG) * d.k : 0);
+        launch(c, "jet_rows_group", 0.0, dim3(grid_for(c, div_up(d.n, SWEEP_PIECE), 1, 4)), dim3(JB), smem,
                row_kernel<GG>, d, init);
     }
     // the rows of the wide vertices next to the last iteration's moves (or, at
     // init, of every wide vertex)
     void row_update_launches(u32 init) {
         if (!rows) return;
-        HEIPA_DISPATCH_G(G, launch_rows, init);
+        if (d.W <= 2)
+            launch(c, "jet_rows", 0.0, dim3(grid_for(c, d.n, JB, 8)), dim3(JB), 0, row_thread_kernel, d, init);
+        if (d.W > 2 || (init && meta.max_deg > ROW_THREAD_DEG)) HEIPA_DISPATCH_G(G, launch_rows, init);
         if (meta.n_hubs)
-            launch(c, "jet_rows", 0.0, dim3(meta.n_hubs), dim3(JB), 4ull * d.W, row_hub_kernel, d,
+            launch(c, "jet_rows_hub", 0.0, dim3(meta.n_hubs), dim3(JB), 4ull * d.W, row_hub_kernel, d,
                    (const u32 *) meta.hubs, init);
     }
 
@@ -3162,12 +3290,16 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         d.rflag = bs.get<u8>(n);
         d.wid = bs.get<u32>(n);
         u32 *cnt = bs.get<u32>(1);
-        scan<u32>(c, "jet_init", WideSrc{g.xadj, k}, WideDst{g.xadj, k, d.wid, d.rflag}, n, cnt);
+        scan<u32>(c, "jet_init", WideSrc{g.xadj, k}, WideDst{g.xadj, k, d.wid, d.rflag, meta.hub_deg}, n, cnt);
         u32 nw = 0;
         readback(c, &nw, cnt, 1);
         d.wsz = bs.get<u32>(nw ? nw : 1);
         d.wV = bs.get<u32>((u64) (nw ? nw : 1) * d.W);
         d.wP = bs.get<u32>((u64) (nw ? nw : 1) * d.W);
+        if (d.W <= 2) {  // counts of the counted vertices; nw * k <= m (a wide vertex has k edges)
+            d.wcnt = bs.get<u32>((u64) (nw ? nw : 1) * k);
+            fill<u32>(c, d.wcnt, (u64) (nw ? nw : 1) * k, 0u);
+        }
     }
     copy_d2d(c, d.work, part, 4ull * n);
     fill<u32>(c, d.dcache, n, NULL_PART);
