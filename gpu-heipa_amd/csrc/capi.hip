@@ -3,6 +3,7 @@
 // codes (the reference exits or aborts instead).
 #include "device/eval.hpp"
 #include "host/host_graph.hpp"
+#include "host/lend.h"
 
 #include <cstdio>
 #include <cstring>
@@ -77,6 +78,7 @@ int heipa_ctx_create(int device, uint64_t arena_bytes, heipa_ctx **out) {
     *out = nullptr;
     heipa_ctx *ctx = new (std::nothrow) heipa_ctx();
     if (!ctx) return HEIPA_ERR_OOM;
+    ctx->c.handle = ctx;
     try {
         ctx_open(ctx->c, device, arena_bytes);
     } catch (const Error &e) {
@@ -445,6 +447,28 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         }
         else require(false, "unknown option");
     });
+}
+
+int heipa_ctx_set_lender_internal(heipa_ctx *ctx, heipa_borrow_fn borrow, heipa_give_back_fn give_back, void *user) {
+    if (!ctx) return HEIPA_ERR_INVALID;
+    if (!borrow || !give_back) {
+        ctx->c.borrow = nullptr;
+        ctx->c.give_back = nullptr;
+        return HEIPA_OK;
+    }
+    ctx->c.borrow = [borrow, user](size_t max) {
+        std::vector<heipa_ctx *> got(max);
+        got.resize(max ? borrow(user, got.data(), max) : 0);
+        std::vector<Ctx *> out;
+        for (heipa_ctx *h: got) out.push_back(&h->c);
+        return out;
+    };
+    ctx->c.give_back = [give_back, user](const std::vector<Ctx *> &lent) {
+        std::vector<heipa_ctx *> hs;
+        for (Ctx *x: lent) hs.push_back(static_cast<heipa_ctx *>(x->handle));
+        give_back(user, hs.data(), hs.size());
+    };
+    return HEIPA_OK;
 }
 
 int heipa_ctx_share_hier_cache(heipa_ctx *dst, const heipa_ctx *src) {

@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -221,6 +222,12 @@ struct Ctx {
     // worker contexts (own stream, arena and staging) for independent
     // subproblems run concurrently, kept between calls
     std::vector<std::unique_ptr<Ctx>> workers;
+    // a pool's idle worker contexts lent to this context's HM subtree solves
+    // (set by the pool for its workers; null elsewhere): borrow(max) returns
+    // up to max of them, give_back returns them to the pool
+    std::function<std::vector<Ctx *>(size_t)> borrow;
+    std::function<void(const std::vector<Ctx *> &)> give_back;
+    void *handle = nullptr;  // the C-ABI object that holds this context
 
     hipEvent_t get_event() {
         if (!event_pool.empty()) {
@@ -342,6 +349,8 @@ inline void ctx_copy_settings(Ctx &to, const Ctx &from) {
     to.use_graphs = from.use_graphs;
     to.bucket_copies = from.bucket_copies;
     to.block_lds = from.block_lds;
+    to.group_cap = from.group_cap;
+    to.block_group_cap = from.block_group_cap;
     to.hm_workers = from.hm_workers;
     to.hm_worker_bytes = from.hm_worker_bytes;
     to.race = from.race;

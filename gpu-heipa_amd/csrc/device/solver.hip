@@ -267,54 +267,79 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
         u32 *n_to_o;
         u32 id;
     };
-    std::vector<Kid> kids;
     const u32 k = p.hier[level];
+    // In a pool (one subtree worker per job), the subtrees run on the pool's
+    // idle worker contexts beside this one; with none idle, on this one alone.
+    std::vector<Ctx *> lent;
+    const bool lending = c.hm_workers <= 1 && c.borrow;
+    if (lending) {
+        lent = c.borrow(k - 1);
+        if (lent.empty()) return false;
+    }
+    struct GiveBack {
+        Ctx &c;
+        std::vector<Ctx *> &lent;
+        ~GiveBack() {
+            if (!lent.empty()) c.give_back(lent);
+        }
+    } give_back{c, lent};
+    std::vector<Kid> kids;
     for (u32 id = 0; id < k; ++id) {
         Kid kd{};
         kd.id = id;
         if (build_child(c, g, tmp, n_to_o, id, kd.g, kd.n_to_o)) kids.push_back(kd);
     }
-    if (kids.size() < 2) {
+    auto sequential = [&] {
         for (const Kid &kd: kids) {
             std::vector<u32> ident{kd.id};
             hm_rec(c, kd.g, kd.n_to_o, level - 1, ident, p, global);
         }
         return true;
-    }
+    };
+    if (kids.size() < 2) return sequential();
     c.sync();  // the children are complete before other streams read them
-    const size_t nw = std::min<size_t>(kids.size(), c.hm_workers);
-    u64 need = 0;
-    for (const Kid &kd: kids) need = std::max(need, hm_child_bytes(kd.g, p, level - 1));
-    if (c.hm_worker_bytes) need = c.hm_worker_bytes;
-    try {
-        if (c.workers.size() < nw) c.workers.resize(nw);
-        for (size_t i = 0; i < nw; ++i) {
-            auto &w = c.workers[i];
-            if (w && w->arena.capacity() >= need) continue;
-            if (w) ctx_close(*w);
-            w = std::make_unique<Ctx>();
-            ctx_open(*w, c.device, need);
+    // W: the contexts that solve the subtrees, kid i on W[i mod nw]
+    std::vector<Ctx *> W;
+    if (lending) {
+        W.push_back(&c);
+        for (Ctx *x: lent) W.push_back(x);
+        while (W.size() > kids.size()) {  // more lent than children: the rest go back now
+            c.give_back({W.back()});
+            lent.pop_back();
+            W.pop_back();
         }
-    } catch (const std::exception &) {  // device or host memory for the workers: one after another here
-        for (auto &w: c.workers)
-            if (w) ctx_close(*w);
-        c.workers.clear();
-        for (const Kid &kd: kids) {
-            std::vector<u32> ident{kd.id};
-            hm_rec(c, kd.g, kd.n_to_o, level - 1, ident, p, global);
+    } else {
+        const size_t nw = std::min<size_t>(kids.size(), c.hm_workers);
+        u64 need = 0;
+        for (const Kid &kd: kids) need = std::max(need, hm_child_bytes(kd.g, p, level - 1));
+        if (c.hm_worker_bytes) need = c.hm_worker_bytes;
+        try {
+            if (c.workers.size() < nw) c.workers.resize(nw);
+            for (size_t i = 0; i < nw; ++i) {
+                auto &w = c.workers[i];
+                if (w && w->arena.capacity() >= need) continue;
+                if (w) ctx_close(*w);
+                w = std::make_unique<Ctx>();
+                ctx_open(*w, c.device, need);
+            }
+        } catch (const std::exception &) {  // device or host memory for the workers: one after another here
+            for (auto &w: c.workers)
+                if (w) ctx_close(*w);
+            c.workers.clear();
+            return sequential();
         }
-        return true;
+        for (size_t i = 0; i < nw; ++i) W.push_back(c.workers[i].get());
     }
-    // workers take the children in turn: kid i runs on worker i mod nw,
+    const size_t nw = W.size();
     // each worker's children one after another on its own stream
     std::vector<std::exception_ptr> errors(nw);
     std::vector<u8> done(kids.size(), 0);
     auto run = [&](size_t wi) {
-        Ctx &w = *c.workers[wi];
+        Ctx &w = *W[wi];
         const size_t fd = w.arena.front_depth(), bd = w.arena.back_depth();
         try {
             HIP_CHECK(hipSetDevice(w.device));
-            ctx_copy_settings(w, c);
+            if (&w != &c) ctx_copy_settings(w, c);
             for (size_t i = wi; i < kids.size(); i += nw) {
                 std::vector<u32> ident{kids[i].id};
                 hm_rec(w, kids[i].g, kids[i].n_to_o, level - 1, ident, p, global);
@@ -330,15 +355,17 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
         }
     };
     std::vector<std::thread> threads;
-    for (size_t wi = 0; wi < nw; ++wi) {
+    for (size_t wi = lending ? 1 : 0; wi < nw; ++wi) {
         try {
             threads.emplace_back(run, wi);
         } catch (const std::system_error &) {
             run(wi);  // no thread available: this worker's children run here
         }
     }
+    if (lending) run(0);  // this context's own share, on the calling thread
     for (auto &t: threads) t.join();
-    for (size_t wi = 0; wi < nw; ++wi) ctx_merge_prof(c, *c.workers[wi]);
+    for (size_t wi = 0; wi < nw; ++wi)
+        if (W[wi] != &c) ctx_merge_prof(c, *W[wi]);
     HIP_CHECK(hipSetDevice(c.device));
     // A worker whose arena (sized by the estimate) ran out hands its
     // unfinished children back: they run one after another on c, whose arena
@@ -349,7 +376,7 @@ bool hm_split_parallel(Ctx &c, const DGraph &g, const u32 *tmp, const u32 *n_to_
         try {
             std::rethrow_exception(errors[wi]);
         } catch (const Error &e) {
-            if (e.code != HEIPA_ERR_OOM) throw;
+            if (e.code != HEIPA_ERR_OOM || W[wi] == &c) throw;
         }
         for (size_t i = wi; i < kids.size(); i += nw) {
             if (done[i]) continue;
@@ -378,7 +405,8 @@ void hm_rec(Ctx &c, const DGraph &g, const u32 *n_to_o, u32 level, std::vector<u
         c.arena.release_front(depth0);
         return;
     }
-    if (ident.empty() && level > 0 && c.hm_workers > 1 && hm_split_parallel(c, g, tmp, n_to_o, level, p, global)) {
+    if (ident.empty() && level > 0 && (c.hm_workers > 1 || c.borrow) &&
+        hm_split_parallel(c, g, tmp, n_to_o, level, p, global)) {
         c.arena.release_front(depth0);
         return;
     }

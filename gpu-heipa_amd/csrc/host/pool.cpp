@@ -11,6 +11,7 @@
 // worker's context: its result is the same alone and in the mix, and the
 // kernels of jobs on different workers overlap on the device.
 #include "../../../include/heipa.h"
+#include "lend.h"
 
 #include <algorithm>
 #include <chrono>
@@ -34,6 +35,7 @@ struct heipa_pool {
         uint32_t passed;  // later jobs admitted before it
     };
     std::deque<Queued> queue;
+    std::vector<uint8_t> idle, lent;  // per worker: waiting for a job / lent to another job's HM subtrees
     uint64_t inflight = 0;
     bool stop = false;
     // the launch-shape options heipa_pool_autotune searches, as last set
@@ -94,6 +96,12 @@ void run_job(heipa_ctx *ctx, heipa_job &j) {
 
 constexpr uint32_t MAX_PASSED = 16;
 
+// An HM job's subtrees below its first split may run on workers that are
+// idle while no queued job can start: they are lent to the job (no queued
+// job starts on them) and come back when its subtrees are done.
+size_t borrow_cb(void *user, heipa_ctx **out, size_t max);
+void give_back_cb(void *user, heipa_ctx *const *ctxs, size_t n);
+
 // index of the job a free worker takes now, or -1: the first queued job that
 // fits the budget, unless an earlier one has been passed MAX_PASSED times
 long admissible(const heipa_pool *p) {
@@ -111,11 +119,14 @@ void worker(heipa_pool *p, uint32_t slot) {
         {
             std::unique_lock<std::mutex> lk(p->m);
             long pick = -1;
+            p->idle[slot] = 1;
             p->cv_work.wait(lk, [&] {
+                if (p->lent[slot]) return false;
                 if (p->stop && p->queue.empty()) return true;
                 pick = admissible(p);
                 return pick >= 0;
             });
+            p->idle[slot] = 0;
             if (pick < 0) return;  // stopping
             job = p->queue[(size_t) pick].job;
             for (long i = 0; i < pick; ++i) p->queue[(size_t) i].passed++;
@@ -141,6 +152,30 @@ void worker(heipa_pool *p, uint32_t slot) {
     }
 }
 
+size_t borrow_cb(void *user, heipa_ctx **out, size_t max) {
+    heipa_pool *p = static_cast<heipa_pool *>(user);
+    std::lock_guard<std::mutex> lk(p->m);
+    if (admissible(p) >= 0) return 0;  // a queued job can start: the idle workers are its
+    size_t n = 0;
+    for (size_t s = 0; s < p->ctxs.size() && n < max; ++s)
+        if (p->idle[s] && !p->lent[s]) {
+            p->lent[s] = 1;
+            out[n++] = p->ctxs[s];
+        }
+    return n;
+}
+
+void give_back_cb(void *user, heipa_ctx *const *ctxs, size_t n) {
+    heipa_pool *p = static_cast<heipa_pool *>(user);
+    {
+        std::lock_guard<std::mutex> lk(p->m);
+        for (size_t i = 0; i < n; ++i)
+            for (size_t s = 0; s < p->ctxs.size(); ++s)
+                if (p->ctxs[s] == ctxs[i]) p->lent[s] = 0;
+    }
+    p->cv_work.notify_all();
+}
+
 } // namespace
 
 extern "C" {
@@ -164,10 +199,13 @@ int heipa_pool_create(int device, uint32_t slots, uint64_t arena_bytes, uint64_t
             return rc;
         }
         // the pool's slots already overlap whole jobs: an HM job solves its
-        // subtrees on its own slot, one after another
+        // subtrees on its own slot and on the slots idle at that moment
         heipa_ctx_set_option(c, HEIPA_OPT_HM_WORKERS, 1);
+        heipa_ctx_set_lender_internal(c, borrow_cb, give_back_cb, p);
         p->ctxs.push_back(c);
     }
+    p->idle.assign(slots, 0);
+    p->lent.assign(slots, 0);
     try {
         for (uint32_t s = 0; s < slots; ++s) p->threads.emplace_back(worker, p, s);
     } catch (...) {
