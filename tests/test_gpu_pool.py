@@ -157,3 +157,28 @@ def test_pool_autotune_keeps_results(ctx):
     finally:
         pool.close()
 
+
+def test_pool_hm_job_on_lent_workers(ctx):
+    """An HM job alone in a pool solves its subtrees below the first split on
+    the pool's idle workers (lent to it while no queued job can start); a
+    second run beside partition jobs lends fewer.  Both equal the alone run."""
+    g = heipa.gen_rmat(14, 0, 16.0, 0.57, 0.19, 0.19, 7, 100, True)
+    dg = _dev(g)
+    want = _alone(ctx, dg, dict(hier=[4, 8, 6]))
+    small = [(heipa.DeviceGraph(heipa.gen_rgg(4000 + 500 * i, 8.0, 20 + i, 10), "cuda:0"), dict(k=8)) for i in range(5)]
+    pool = heipa.Pool(6, 512 << 20)
+    try:
+        j = heipa.Pool.job(dg, hier=[4, 8, 6], imbalance=0.03, seed=1)
+        pool.run([j])
+        assert j.status == heipa.OK
+        assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], want)
+        jobs = [heipa.Pool.job(dg, hier=[4, 8, 6], imbalance=0.03, seed=1)] + \
+               [heipa.Pool.job(d, imbalance=0.03, seed=1, **s) for d, s in small]
+        pool.run(jobs)
+        assert all(x.status == heipa.OK for x in jobs)
+        assert np.array_equal(heipa.to_numpy_u32(jobs[0]._part)[:dg.n], want)
+        for (d, s), x in zip(small, jobs[1:]):
+            assert np.array_equal(heipa.to_numpy_u32(x._part)[:d.n], _alone(ctx, d, s))
+    finally:
+        pool.close()
+
