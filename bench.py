@@ -108,8 +108,8 @@ def main():
                                                       "to PATH.batched.json / PATH.served.json")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path does not shard "
-              f"across GPUs in this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path runs on one "
+              f"GPU; independent graphs and requests would shard one context per GPU)", file=sys.stderr)
         sys.exit(2)
     if WORKLOADS[args.workload]["kind"] == "headline":
         return run_headline(args)
