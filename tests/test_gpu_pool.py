@@ -182,3 +182,25 @@ def test_pool_hm_job_on_lent_workers(ctx):
     finally:
         pool.close()
 
+
+def test_pool_autotune_reports_a_failed_sample_job(ctx):
+    """A malformed graph in the autotuner's sample ends the tuning with that
+    job's status; the pool keeps working afterwards."""
+    reqs = _requests()[:3]
+    bad = heipa.HostGraph(np.array([0, 1, 2], np.uint32), np.array([1, 7], np.uint32))
+    bad_dg = heipa.DeviceGraph(bad, "cuda:0")
+    pool = heipa.Pool(2, 512 << 20)
+    try:
+        sample = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs]
+        sample.append(heipa.Pool.job(bad_dg, k=2, imbalance=0.03, seed=1))
+        with pytest.raises(heipa.HeipaError) as e:
+            pool.autotune(sample)
+        assert e.value.code == heipa.ERR_INVALID
+        jobs = [heipa.Pool.job(dg, imbalance=0.03, seed=1, **s) for dg, s in reqs]
+        pool.run(jobs)
+        for (dg, s), j in zip(reqs, jobs):
+            assert j.status == heipa.OK
+            assert np.array_equal(heipa.to_numpy_u32(j._part)[:dg.n], _alone(ctx, dg, s)), s
+    finally:
+        pool.close()
+
