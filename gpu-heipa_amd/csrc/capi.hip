@@ -427,6 +427,7 @@ int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value) {
         else if (option == HEIPA_OPT_BLOCK_LDS) c.block_lds = value != 0;
         else if (option == HEIPA_OPT_GROUP_CAP) c.group_cap = (u32) std::min<uint64_t>(value, 64);
         else if (option == HEIPA_OPT_BLOCK_GROUP_CAP) c.block_group_cap = (u32) std::min<uint64_t>(value, 64);
+        else if (option == HEIPA_OPT_REGION_EDGES) c.region_edges = (u32) std::min<uint64_t>(value, 0xFFFFFFFFull);
         else if (option == HEIPA_OPT_HM_WORKERS) {
             c.hm_workers = (u32) std::min<uint64_t>(value, 64);
             ctx_trim_workers(c, c.hm_workers > 1 ? c.hm_workers : 0);  // frees the arenas of the ones dropped

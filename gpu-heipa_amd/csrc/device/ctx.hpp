@@ -190,6 +190,7 @@ struct Ctx {
     u32 block_jet_max_n = 2048;  // HEIPA_OPT_BLOCK_JET
     u32 group_cap = 64;         // HEIPA_OPT_GROUP_CAP
     u32 block_group_cap = 8;    // HEIPA_OPT_BLOCK_GROUP_CAP
+    u32 region_edges = 1024;    // HEIPA_OPT_REGION_EDGES
     bool use_graphs = true;  // HEIPA_OPT_GRAPHS
     u32 bucket_copies = 0;   // HEIPA_OPT_BUCKET_COPIES (0: automatic)
     bool block_lds = true;   // HEIPA_OPT_BLOCK_LDS
@@ -351,6 +352,7 @@ inline void ctx_copy_settings(Ctx &to, const Ctx &from) {
     to.block_lds = from.block_lds;
     to.group_cap = from.group_cap;
     to.block_group_cap = from.block_group_cap;
+    to.region_edges = from.region_edges;
     to.hm_workers = from.hm_workers;
     to.hm_worker_bytes = from.hm_worker_bytes;
     to.race = from.race;

@@ -1305,6 +1305,17 @@ __device__ __forceinline__ u32 chunk_of(const s64 *es, u32 k, s64 pos, s64 wu, u
 // later in a chunk that has no room for the first have less, so the loop
 // jumps to the first position at or past the next chunk's start: the same
 // position and counter as one take at a time, in at most one step per chunk.
+// the smallest multiple of wu at or above dist (both positive); 32-bit
+// division where both fit 31 bits (the 64-bit one is a long instruction
+// sequence, and the overflow step runs it serially)
+__device__ __forceinline__ s64 round_up_to(s64 dist, s64 wu) {
+    if (((u64) dist | (u64) wu) < (1ull << 31)) {
+        const u32 q = ((u32) dist + (u32) wu - 1u) / (u32) wu;
+        return (s64) q * wu;
+    }
+    return (dist + wu - 1) / wu * wu;
+}
+
 template<typename P>
 __device__ __forceinline__ u32 overflow_chunk(P es, u32 k, s64 pos, s64 wu, s64 &counter) {
     s64 id = chunk_index(es, k, pos);
@@ -1319,7 +1330,7 @@ __device__ __forceinline__ u32 overflow_chunk(P es, u32 k, s64 pos, s64 wu, s64 
             break;
         }
         const s64 next = es[id + 1];  // > pos; es[0] when id = -1
-        pos += (next - pos + wu - 1) / wu * wu;
+        pos += round_up_to(next - pos, wu);
     }
     counter = pos + wu;
     return id >= (s64) k ? k : (u32) id;
@@ -1333,8 +1344,10 @@ __device__ __forceinline__ lds_s64 *les_of(const s64 *p) { return (lds_s64 *) p;
 
 // cookie cutter (jet:508-552), by the first wave of the workgroup; on entry
 // evict_adjust holds the filter's values, on exit the eviction position of
-// each block's first candidate, so that position = score + evict_adjust[block]
-__device__ void cookie_cutter(const JetDev &d, s64 mbw) {
+// each block's first candidate, so that position = score + evict_adjust[block].
+// With les, the chunk bounds also go to that LDS array; with lcnt, the
+// overflow counter's start goes there instead of to device memory.
+__device__ void cookie_cutter(const JetDev &d, s64 mbw, s64 *les = nullptr, s64 *lcnt = nullptr) {
     const u32 lane = threadIdx.x;
     s64 carry = 0, carry2 = 0;
     for (u32 b0 = 0; b0 < d.k; b0 += WAVE) {
@@ -1349,13 +1362,16 @@ __device__ void cookie_cutter(const JetDev &d, s64 mbw) {
         if (p < d.k) {
             d.evict_adjust[p] = carry + inc - add;
             d.evict_start[p + 1] = carry2 + inc2;
+            if (les) les[p + 1] = carry2 + inc2;
         }
         carry += __shfl(inc, WAVE - 1, WAVE);
         carry2 += __shfl(inc2, WAVE - 1, WAVE);
     }
     if (lane == 0) {
-        *d.counter = carry;
+        if (lcnt) *lcnt = carry;
+        else *d.counter = carry;
         d.evict_start[0] = 0;
+        if (les) les[0] = 0;
     }
 }
 
@@ -1370,6 +1386,7 @@ struct ResolveLds {
     u8 *ssel;
     s64 *red;    // RB / WAVE
     u32 *ocount; // [0] overflow entries, [1] the first that takes the counter twice
+    s64 *cnt;    // the overflow counter during one resolve step
     u32 *olist;  // OVF_CAP
     u32 *ovid;   // OVF_CAP
     u32 *oent;   // OVF_CAP
@@ -1389,6 +1406,7 @@ __device__ inline ResolveLds resolve_lds(char *base, u32 cap) {
     r.red = (s64 *) p;
     p += 8ull * (RB / WAVE);
     r.ocount = (u32 *) p;
+    r.cnt = (s64 *) (p + 8);
     p += 16;
     r.olist = (u32 *) p;
     p += 4ull * OVF_CAP;
@@ -1429,9 +1447,10 @@ __device__ __forceinline__ void overflow_block(const JetDev &d, P es, const Reso
     const bool act = t < oc;
     const u32 e = act ? lds.oent[t] : 0;
     const s64 w = act ? (s64) (skey[e] >> 32) : 0;
+    if (act) lds.ovid[t] = (u32) w;  // the weights in rank order, for the serial part
     s64 tot;
     const s64 ex = block_exclusive_scan<NT>(w, lds.red, tot);
-    const s64 c0 = *d.counter;
+    const s64 c0 = *lds.cnt;
     const s64 pos = c0 + ex;
     u32 id = k;
     if (act) {
@@ -1452,7 +1471,7 @@ __device__ __forceinline__ void overflow_block(const JetDev &d, P es, const Reso
         s64 cid = first < oc ? chunk_index(es, k, counter) : 0;
         for (u32 r = first; r < oc; ++r) {
             const u32 i = lds.oent[r];
-            const s64 wu = (s64) (skey[i] >> 32);
+            const s64 wu = (s64) lds.ovid[r];
             s64 pos = counter;
             u32 got;
             while (true) {
@@ -1469,12 +1488,11 @@ __device__ __forceinline__ void overflow_block(const JetDev &d, P es, const Reso
                     got = k;
                     break;
                 }
-                pos += (es[cid + 1] - pos + wu - 1) / wu * wu;  // first take at or past the next chunk
+                pos += round_up_to(es[cid + 1] - pos, wu);  // first take at or past the next chunk
             }
             soff[i] = got;
             counter = pos + wu;
         }
-        *d.counter = counter;
     }
 }
 
@@ -1599,11 +1617,8 @@ __device__ __forceinline__ bool resolve_body(const JetDev &d, bool strong, u32 s
     }
     __syncthreads();
     if (strong) {
-        if (threadIdx.x < WAVE) cookie_cutter(d, max_block_weight(d, true));
-        __syncthreads();
         const bool es_lds = d.k + 1 <= ES_CAP;
-        if (es_lds)
-            for (u32 i = threadIdx.x; i <= d.k; i += NT) s_es[i] = d.evict_start[i];
+        if (threadIdx.x < WAVE) cookie_cutter(d, max_block_weight(d, true), es_lds ? s_es : nullptr, lds.cnt);
         __syncthreads();
         const s64 *es = es_lds ? s_es : d.evict_start;
         // adjust_scores, first pass; overflowing vertices are resolved below
@@ -2389,7 +2404,7 @@ __global__ __launch_bounds__(JB) void row_thread_kernel(JetDev d, u32 init) {
 // the flagged wide vertices that are not hubs, G lanes each; init: every
 // wide vertex (flagged by the set-up), before the loop starts
 template<int G>
-__global__ __launch_bounds__(JB) void row_kernel(JetDev d, u32 init) {
+__global__ __launch_bounds__(JB) void row_kernel(JetDev d, u32 init, u32 piece) {
     extern __shared__ __align__(16) u32 rsm[];
     __shared__ u32 s_nlist;
     u32 rmode = ROW_INIT;
@@ -2403,8 +2418,8 @@ __global__ __launch_bounds__(JB) void row_kernel(JetDev d, u32 init) {
     u32 *wm = rsm + SWEEP_PIECE + gi * d.W;
     // the counted vertices' counts at init: k LDS counters per group (k <= 64)
     u32 *cl = init && d.W <= 2 ? rsm + SWEEP_PIECE + groups * d.W + gi * d.k : nullptr;
-    for (u64 p0 = (u64) blockIdx.x * SWEEP_PIECE; p0 < d.n; p0 += (u64) gridDim.x * SWEEP_PIECE) {
-        const u64 p1 = p0 + SWEEP_PIECE < d.n ? p0 + SWEEP_PIECE : d.n;
+    for (u64 p0 = (u64) blockIdx.x * piece; p0 < d.n; p0 += (u64) gridDim.x * piece) {
+        const u64 p1 = p0 + piece < d.n ? p0 + piece : d.n;
         if (threadIdx.x == 0) s_nlist = 0;
         __syncthreads();
         for (u64 v0 = p0; v0 < p1; v0 += JB) {
@@ -2471,10 +2486,16 @@ __host__ __device__ constexpr size_t bm_state_bytes(u32 n, u32 k) { return 50ull
 // memory before the kernel returns (the host's global resolve path and the
 // next launch read it there).  The device functions reach it through the
 // same pointers in a local copy of the Jet state, as generic addresses.
+// The control block is kept in LDS the same way in both forms.
 template<int G, bool LS>
 __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters, u32 groups, u32 state_off) {
     extern __shared__ __align__(16) char smem[];
+    constexpr u32 CW = (sizeof(JetCtl) + 7) / 8;
+    static_assert(CW <= BM && sizeof(JetCtl) % 8 == 0, "control block copy");
+    __shared__ u64 s_ctl[CW];
     JetDev d = dg;
+    if (threadIdx.x < CW) s_ctl[threadIdx.x] = ((const u64 *) dg.ctl)[threadIdx.x];
+    d.ctl = (JetCtl *) s_ctl;
     if constexpr (LS) {
         const u32 n = dg.n, k = dg.k;
         s64 *gc8 = (s64 *) (smem + state_off);
@@ -2504,8 +2525,8 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             d.lock[n + v] = dg.lock[n + v];  // stale bytes
         }
         for (u32 b = threadIdx.x; b < k; b += BM) d.bw[b] = dg.bw[b];
-        __syncthreads();
     }
+    __syncthreads();
     __shared__ u32 s_live, s_mode, s_stamp, s_save;
     __shared__ double s_ratio;
     __shared__ u32 s_ncand, s_nlist, s_nmoves, s_nrl;
@@ -2789,8 +2810,9 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             __syncthreads();
         }
     }
+    __syncthreads();
+    if (threadIdx.x < CW) ((u64 *) dg.ctl)[threadIdx.x] = s_ctl[threadIdx.x];
     if constexpr (LS) {
-        __syncthreads();
         for (u32 v = threadIdx.x; v < dg.n; v += BM) {
             dg.gcache[v] = d.gcache[v];
             dg.gain1[v] = d.gain1[v];
@@ -3009,10 +3031,14 @@ struct JetHost {
     JetHost(Ctx &cc, const DGraph &gg, const GraphMeta &mm) : c(cc), g(gg), meta(mm) {}
 
     template<int GG>
+    // pieces of at most SWEEP_PIECE vertices (the LDS list), and no larger than
+    // a list region: on dense levels a region holds few vertices, and one
+    // workgroup per 2048 vertices walked every wide vertex of a small level
     void launch_rows(u32 init) {
         const size_t smem = 4ull * SWEEP_PIECE + 4ull * (JB / GG) * d.W + (init && d.W <= 2 ? 4ull * (JB / GG) * d.k : 0);
-        launch(c, "jet_rows_group", 0.0, dim3(grid_for(c, div_up(d.n, SWEEP_PIECE), 1, 4)), dim3(JB), smem,
-               row_kernel<GG>, d, init);
+        const u32 piece = std::min<u32>(SWEEP_PIECE, d.cap);
+        launch(c, "jet_rows_group", 0.0, dim3(grid_for(c, div_up(d.n, piece), 1, 4)), dim3(JB), smem,
+               row_kernel<GG>, d, init, piece);
     }
     // the rows of the wide vertices next to the last iteration's moves (or, at
     // init, of every wide vertex)
@@ -3252,13 +3278,21 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
                 "k too large for the LDS block-connectivity counters");
     }
     // list regions: sweep workgroup r owns vertices [r * cap, (r + 1) * cap),
-    // cap a multiple of the workgroup size; its post workgroup may add its
+    // cap a multiple of the region unit; its post workgroup may add its
     // share of the extra candidate list (hubs) to its move region.  At most 4
     // regions per CU: every sweep workgroup is then resident at once (its LDS
     // allows about 6 per CU), and on the RGG 2^20 k = 64 level 0 the sweep
-    // took 24 us against 28 us with 8 per CU (2 per CU: 31 us, 16: 30 us)
+    // took 24 us against 28 us with 8 per CU (2 per CU: 31 us, 16: 30 us).
+    // The unit is the sweep workgroup's thread count, halved on dense levels
+    // while it holds more than region_edges adjacency entries on average: a
+    // workgroup's lane groups take its vertices a few at a time, and on the
+    // dense coarse levels of skewed graphs few large regions left most of the
+    // chip idle while each workgroup walked hundreds of long adjacencies
     {
-        const u64 per_block = h.sweep_threads;
+        u64 per_block = h.sweep_threads;
+        const double avg_deg = (double) g.m / (double) n;
+        while (c.region_edges && per_block > 8 && (double) per_block * avg_deg > (double) c.region_edges)
+            per_block >>= 1;
         d.nreg = grid_for(c, n, (u32) per_block, 4);
         d.cap = (u32) (div_up(n, (u64) d.nreg * per_block) * per_block);
         d.nreg = div_up(n, d.cap);

@@ -167,13 +167,15 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long *ctr, u64 v) {
 // is kept for the next round; kept vertices collect in LDS and each workgroup
 // reserves their place in out_list with one counter update per buffer fill
 // (same-address atomics from every wave serialised the former form).  Hubs
-// were repicked by pick_hub_kernel before this launch.
+// were repicked by pick_hub_kernel before this launch.  A workgroup takes
+// pieces of `piece` <= MB entries: small pieces on dense levels, where one
+// workgroup's lane groups otherwise walked many long adjacencies in turn.
 constexpr u32 RP_KEEP = 2048;
 template<int G, bool UEW>
 __global__ __launch_bounds__(MB) void repick_kernel(DGraph g, u32 *__restrict__ vcmap, u32 *__restrict__ hn, u32 seed,
                                                     const u32 *__restrict__ list, const u32 *__restrict__ len_ptr,
                                                     u32 hub_deg, u32 *__restrict__ out_list, u32 *__restrict__ out_len,
-                                                    unsigned long long *deg_acc) {
+                                                    unsigned long long *deg_acc, u32 piece) {
     __shared__ u32 s_work[MB];
     __shared__ u32 s_keep[RP_KEEP];
     __shared__ u32 s_nwork, s_nkeep, s_base;
@@ -190,13 +192,13 @@ __global__ __launch_bounds__(MB) void repick_kernel(DGraph g, u32 *__restrict__ 
         if (threadIdx.x == 0) s_nkeep = 0;
         __syncthreads();
     };
-    for (u64 p0 = (u64) blockIdx.x * MB; p0 < len; p0 += (u64) gridDim.x * MB) {
+    for (u64 p0 = (u64) blockIdx.x * piece; p0 < len; p0 += (u64) gridDim.x * piece) {
         if (threadIdx.x == 0) s_nwork = 0;
         __syncthreads();
         const u64 i = p0 + threadIdx.x;
         u32 u = 0, h = SENT;
         bool unmatched = false, need = false;
-        if (i < len) {
+        if (threadIdx.x < piece && i < len) {
             u = list ? list[i] : (u32) i;
             unmatched = vcmap[u] == SENT;
             h = hn[u];
@@ -482,13 +484,17 @@ void launch_pick(Ctx &c, const DGraph &g, const GraphMeta &meta, bool initial, u
             launch(c, "match_initial_pick", bytes, grid, block, 0, pick_kernel<G, true, false>, g, vcmap, hn, seed,
                    list, len_ptr, meta.hub_deg, out_list, out_len);
     } else {
-        const dim3 grid(grid_for(c, len_bound, MB, 2));
+        // pieces halve while they hold more than region_edges adjacency entries
+        u32 piece = MB;
+        const double avg_deg = g.n ? (double) g.m / (double) g.n : 0.0;
+        while (c.region_edges && piece > 8 && (double) piece * avg_deg > (double) c.region_edges) piece >>= 1;
+        const dim3 grid(grid_for(c, len_bound, piece, 2));
         if (uew)
             launch(c, "match_repick", 0.0, grid, block, 0, repick_kernel<G, true>, g, vcmap, hn, seed, list, len_ptr,
-                   meta.hub_deg, out_list, out_len, deg_acc);
+                   meta.hub_deg, out_list, out_len, deg_acc, piece);
         else
             launch(c, "match_repick", 0.0, grid, block, 0, repick_kernel<G, false>, g, vcmap, hn, seed, list,
-                   len_ptr, meta.hub_deg, out_list, out_len, deg_acc);
+                   len_ptr, meta.hub_deg, out_list, out_len, deg_acc, piece);
     }
 }
 

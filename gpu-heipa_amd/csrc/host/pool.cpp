@@ -327,10 +327,11 @@ int heipa_pool_autotune(heipa_pool *p, heipa_job *sample, uint32_t count, heipa_
     double best = run();
     out->ms_start = best;
     for (Knob &kb: knobs) {
-        uint32_t chosen = *kb.cur;
+        const uint32_t start = *kb.cur;  // timed already: the best run so far
+        uint32_t chosen = start;
         for (uint32_t v: kb.cand) {
             if (status != HEIPA_OK) break;
-            if (v == chosen) continue;
+            if (v == start) continue;
             const int rc = heipa_pool_set_option(p, kb.option, v);
             if (rc != HEIPA_OK) return rc;
             const double t = run();

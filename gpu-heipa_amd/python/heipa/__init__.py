@@ -33,6 +33,7 @@ OPT_HIER_CACHE = 10
 OPT_GRAPH_KEY = 11
 OPT_GROUP_CAP = 12
 OPT_BLOCK_GROUP_CAP = 13
+OPT_REGION_EDGES = 14
 UNIFORM_VW, UNIFORM_EW = 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -310,6 +310,13 @@ int heipa_pool_hier_cache_stats(const heipa_pool *pool, uint64_t *entries, uint6
  * in the one workgroup).  heipa_pool_autotune picks both for a pool. */
 #define HEIPA_OPT_GROUP_CAP 12
 #define HEIPA_OPT_BLOCK_GROUP_CAP 13
+/* HEIPA_OPT_REGION_EDGES: the multi-launch Jet gives each sweep workgroup a region of the level's
+ * vertices (its candidate and move lists); on a dense level the regions shrink by halves from the
+ * workgroup's thread count (to at least 8 vertices) while a region's vertices times the level's
+ * average degree exceed `value`, so that more workgroups share the level (default 1024; 0 keeps one
+ * vertex per thread).  The row updates of wide vertices and the matching's repick passes size their
+ * pieces the same way.  Results do not depend on it. */
+#define HEIPA_OPT_REGION_EDGES 14
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 /* dst uses src's hierarchy cache (HEIPA_OPT_HIER_CACHE) from now on: one cache for several contexts */
 int heipa_ctx_share_hier_cache(heipa_ctx *dst, const heipa_ctx *src);

@@ -119,6 +119,19 @@ def test_match_contract_coarse_levels(ctx, name):
     test_contract_project(ctx, (name, g))
 
 
+@pytest.mark.parametrize("region_edges", [0, 16])
+@pytest.mark.parametrize("name", sorted(COARSE))
+def test_match_repick_pieces(ctx, name, region_edges):
+    """Repick pieces of a full workgroup (0) or of 8 list entries (16) give
+    the same matching on the coarse levels."""
+    g = COARSE[name]()
+    ctx.set_option(heipa.OPT_REGION_EDGES, region_edges)
+    try:
+        test_match(ctx, (name, g))
+    finally:
+        ctx.set_option(heipa.OPT_REGION_EDGES, 1024)
+
+
 def _trace_tuple(t):
     return (t.ratio_idx, t.iteration, t.mode, t.n_cand, t.n_moves, t.cut, t.maxw, t.moves_hash, t.gain_hash)
 
@@ -243,6 +256,21 @@ def test_jet_bucket_copies(ctx, graph, copies, ultra, level, skew, k):
         test_jet(ctx, graph, ultra, level, skew, k)
     finally:
         ctx.set_option(heipa.OPT_BUCKET_COPIES, 0)
+        ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
+
+
+@pytest.mark.parametrize("region_edges", [0, 16])
+@pytest.mark.parametrize("ultra,level,skew,k", [(False, 0, 0.6, 8), (True, 2, 0.9, 8), (True, 0, 0.3, 64)])
+def test_jet_region_sizes(ctx, graph, region_edges, ultra, level, skew, k):
+    """List regions of one vertex per sweep thread (0) or of 8 vertices (16
+    adjacency entries per region forces the floor on every level) in the
+    multi-launch form give the same iterations."""
+    ctx.set_option(heipa.OPT_REGION_EDGES, region_edges)
+    ctx.set_option(heipa.OPT_BLOCK_JET, 0)
+    try:
+        test_jet(ctx, graph, ultra, level, skew, k)
+    finally:
+        ctx.set_option(heipa.OPT_REGION_EDGES, 1024)
         ctx.set_option(heipa.OPT_BLOCK_JET, 2048)
 
 
