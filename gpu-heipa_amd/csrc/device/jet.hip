@@ -241,12 +241,14 @@ constexpr u32 ROW_THREAD_DEG = 32;  // wide vertices up to this degree walk thei
 // (block_conn.h:208-345): its row update then reads k counts instead of its
 // adjacency.  Vertices of low degree are cheaper to walk, and a hub's counts
 // would serialise the moves' atomics on k addresses, so neither is counted.
-__device__ __forceinline__ void row_touch(const JetDev &d, u32 v, u32 ou, u32 nu) {
-    const u8 f = d.rflag ? d.rflag[v] : (u8) 0;
+// f = rflag[v] and wid = wid[v] (read only if v is counted), loaded by the
+// caller together with the neighbour's other state: a load after the stores
+// of the previous neighbour waited for them
+__device__ __forceinline__ void row_touch(const JetDev &d, u32 v, u8 f, u32 wid, u32 ou, u32 nu) {
     if (f & 1u) {
         if (!(f & 2u)) d.rflag[v] = f | 2u;  // most neighbours of a move list are touched many times
         if ((f & 4u) && ou != nu) {
-            const u64 base = (u64) d.wid[v] * d.k;
+            const u64 base = (u64) wid * d.k;
             atomicSub(&d.wcnt[base + ou], 1u);
             atomicAdd(&d.wcnt[base + nu], 1u);
         }
@@ -1733,7 +1735,8 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
     const s32 *Dn = d.D ? d.D + (u64) nu * d.k : nullptr, *Do = d.D ? d.D + (u64) ou * d.k : nullptr;
     s64 delta = 0;
     for (u64 j0 = b + lane; j0 < e; j0 += (u64) stride * ADJ_UNROLL) {
-        u32 v[ADJ_UNROLL], mk[ADJ_UNROLL], op[ADJ_UNROLL], ds[ADJ_UNROLL], wk[ADJ_UNROLL];
+        u32 v[ADJ_UNROLL], mk[ADJ_UNROLL], op[ADJ_UNROLL], ds[ADJ_UNROLL], wk[ADJ_UNROLL], wi[ADJ_UNROLL];
+        u8 rf[ADJ_UNROLL];
         s64 w[ADJ_UNROLL];
 #pragma unroll
         for (u32 q = 0; q < ADJ_UNROLL; ++q) {
@@ -1748,6 +1751,8 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
             op[q] = ok ? d.oldp[v[q]] : 0u;
             ds[q] = ok ? d.dest[v[q]] : 0u;
             wk[q] = ok ? d.work[v[q]] : 0u;
+            rf[q] = ok && d.rflag ? d.rflag[v[q]] : (u8) 0;
+            wi[q] = ok && d.wcnt ? d.wid[v[q]] : 0u;
         }
 #pragma unroll
         for (u32 q = 0; q < ADJ_UNROLL; ++q) {
@@ -1768,7 +1773,7 @@ __device__ __forceinline__ s64 move_delta_range(const JetDev &d, u32 u, u64 b, u
                 delta += mv ? x : 2 * x;
             }
             stale_of(d)[v[q]] = 3;
-            row_touch(d, v[q], ou, nu);
+            row_touch(d, v[q], rf[q], wi[q], ou, nu);
         }
     }
     return delta;
@@ -2134,6 +2139,8 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
         const u32 E = flat_prefix(L, cnt, deg);
         for (u32 y0 = t; y0 < E; y0 += JB * ADJ_UNROLL) {
             u32 iv[ADJ_UNROLL], v[ADJ_UNROLL], mk[ADJ_UNROLL], op[ADJ_UNROLL], ds[ADJ_UNROLL], wk[ADJ_UNROLL];
+            u32 wi[ADJ_UNROLL];
+            u8 rf[ADJ_UNROLL];
             s64 w[ADJ_UNROLL];
 #pragma unroll
             for (u32 q = 0; q < ADJ_UNROLL; ++q) {
@@ -2151,6 +2158,8 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
                 op[q] = ok ? d.oldp[v[q]] : 0u;
                 ds[q] = ok ? d.dest[v[q]] : 0u;
                 wk[q] = ok ? d.work[v[q]] : 0u;
+                rf[q] = ok && d.rflag ? d.rflag[v[q]] : (u8) 0;
+                wi[q] = ok && d.wcnt ? d.wid[v[q]] : 0u;
             }
 #pragma unroll
             for (u32 q = 0; q < ADJ_UNROLL; ++q) {
@@ -2169,7 +2178,7 @@ __global__ __launch_bounds__(JB) void apply_kernel(JetDev d) {
                     acc += mv ? x : 2 * x;
                 }
                 stale_of(d)[v[q]] = 3;
-                row_touch(d, v[q], ou, nu);
+                row_touch(d, v[q], rf[q], wi[q], ou, nu);
             }
         }
         if (t < cnt && !hub) {
