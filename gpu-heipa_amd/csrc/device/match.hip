@@ -484,10 +484,16 @@ void launch_pick(Ctx &c, const DGraph &g, const GraphMeta &meta, bool initial, u
             launch(c, "match_initial_pick", bytes, grid, block, 0, pick_kernel<G, true, false>, g, vcmap, hn, seed,
                    list, len_ptr, meta.hub_deg, out_list, out_len);
     } else {
-        // pieces halve while they hold more than region_edges adjacency entries
+        // pieces halve while they hold more than region_edges adjacency
+        // entries and the grid is not yet at its cap (past it, smaller pieces
+        // only add per-piece barriers: on the RGG 2^20 levels with 16-lane
+        // groups, 64 rather than 256 entries per piece made these launches
+        // 2.4x slower)
         u32 piece = MB;
         const double avg_deg = g.n ? (double) g.m / (double) g.n : 0.0;
-        while (c.region_edges && piece > 8 && (double) piece * avg_deg > (double) c.region_edges) piece >>= 1;
+        while (c.region_edges && piece > 8 && (double) piece * avg_deg > (double) c.region_edges &&
+               (u64) div_up(len_bound, piece) < (u64) c.num_cus * 2)
+            piece >>= 1;
         const dim3 grid(grid_for(c, len_bound, piece, 2));
         if (uew)
             launch(c, "match_repick", 0.0, grid, block, 0, repick_kernel<G, true>, g, vcmap, hn, seed, list, len_ptr,

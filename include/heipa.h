@@ -315,7 +315,7 @@ int heipa_pool_hier_cache_stats(const heipa_pool *pool, uint64_t *entries, uint6
  * workgroup's thread count (to at least 8 vertices) while a region's vertices times the level's
  * average degree exceed `value`, so that more workgroups share the level (default 1024; 0 keeps one
  * vertex per thread).  The row updates of wide vertices and the matching's repick passes size their
- * pieces the same way.  Results do not depend on it. */
+ * pieces the same way (the repick only while that adds workgroups).  Results do not depend on it. */
 #define HEIPA_OPT_REGION_EDGES 14
 int heipa_ctx_set_option(heipa_ctx *ctx, int option, uint64_t value);
 /* dst uses src's hierarchy cache (HEIPA_OPT_HIER_CACHE) from now on: one cache for several contexts */
