@@ -2062,10 +2062,10 @@ __global__ __launch_bounds__(JB) void apply_groups_kernel(JetDev d) {
     const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
     const u32 *region = d.moves + (u64) blockIdx.x * d.cap_mv;
     // issued with the control block (one round trip): the region length, the
-    // group's first entry (the region holds at least `groups` entries) and the
-    // extra list's length
+    // group's first entry (a region of a dense level may hold fewer than
+    // `groups` entries) and the extra list's length
     const u32 len = d.moves_n[blockIdx.x];
-    const u32 first = region[gi];
+    const u32 first = gi < d.cap_mv ? region[gi] : 0u;
     const u32 lx = (u32) ctl->st[ST_MOVES];
     const CtlHead hd = ctl_head(ctl);
     if (!live(hd)) return;

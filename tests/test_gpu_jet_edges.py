@@ -182,15 +182,18 @@ def test_jet_large_k(ctx, k, ultra, level, form):
     _with_form(ctx, form, run)
 
 
-@pytest.mark.parametrize("n", [2300, 2340, 2370, 2371, 2372, 2450])
+@pytest.mark.parametrize("n", [2200, 2268, 2269, 2282, 2290, 2450])
 def test_jet_block_lds_boundary(ctx, n):
     """The single-workgroup form keeps a level's state in LDS only if the
     state, the dynamic work areas and the kernel's own __shared__ variables
     fit the 160 KiB together (the static part once pushed a launch over the
-    limit).  With the form allowed for any n and k = 8, the state of 2300
-    vertices fits with room, 2370-2372 would have crossed the limit by the
-    kernel's static LDS alone, and 2450 does not fit; every result equals the
-    oracle's."""
+    limit).  With the form allowed for any n and k = 8, the work areas take
+    49,312 bytes (the resolve step's), the state 50 n + 64 and the static
+    part is counted as at least 1 KiB: 2200 vertices fit with room, 2268 is
+    the largest n that keeps its state in LDS, 2269-2282 would still fit the
+    kernel's actual static LDS, 2283-2289 would have crossed the limit by the
+    static part alone, and from 2290 on the dynamic part alone does not fit;
+    every result equals the oracle's."""
     g = heipa.gen_rgg(n, 8.0, 3, 10)
     k = 8
     part0 = random_partition(g.n, k, 2, 1.0)
