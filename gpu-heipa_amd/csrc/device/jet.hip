@@ -1300,13 +1300,6 @@ __device__ __forceinline__ u32 chunk_of(const s64 *es, u32 k, s64 pos, s64 wu, u
     return ub;
 }
 
-// the serial adjust_scores loop of one overflowing vertex (jet:563-585): its
-// chunk, or k when the vertex stays in its block.  After its own position the
-// vertex takes the counter's positions, counter, counter + wu, ..., until one
-// lands in a chunk with room for wu / 2 (or past the last chunk).  Positions
-// later in a chunk that has no room for the first have less, so the loop
-// jumps to the first position at or past the next chunk's start: the same
-// position and counter as one take at a time, in at most one step per chunk.
 // the smallest multiple of wu at or above dist (both positive); 32-bit
 // division where both fit 31 bits (the 64-bit one is a long instruction
 // sequence, and the overflow step runs it serially)
@@ -1318,6 +1311,13 @@ __device__ __forceinline__ s64 round_up_to(s64 dist, s64 wu) {
     return (dist + wu - 1) / wu * wu;
 }
 
+// the serial adjust_scores loop of one overflowing vertex (jet:563-585): its
+// chunk, or k when the vertex stays in its block.  After its own position the
+// vertex takes the counter's positions, counter, counter + wu, ..., until one
+// lands in a chunk with room for wu / 2 (or past the last chunk).  Positions
+// later in a chunk that has no room for the first have less, so the loop
+// jumps to the first position at or past the next chunk's start: the same
+// position and counter as one take at a time, in at most one step per chunk.
 template<typename P>
 __device__ __forceinline__ u32 overflow_chunk(P es, u32 k, s64 pos, s64 wu, s64 &counter) {
     s64 id = chunk_index(es, k, pos);
@@ -3039,10 +3039,10 @@ struct JetHost {
 
     JetHost(Ctx &cc, const DGraph &gg, const GraphMeta &mm) : c(cc), g(gg), meta(mm) {}
 
-    template<int GG>
     // pieces of at most SWEEP_PIECE vertices (the LDS list), and no larger than
     // a list region: on dense levels a region holds few vertices, and one
-    // workgroup per 2048 vertices walked every wide vertex of a small level
+    // workgroup per 1024 vertices walked every wide vertex of a small level
+    template<int GG>
     void launch_rows(u32 init) {
         const size_t smem = 4ull * SWEEP_PIECE + 4ull * (JB / GG) * d.W + (init && d.W <= 2 ? 4ull * (JB / GG) * d.k : 0);
         const u32 piece = std::min<u32>(SWEEP_PIECE, d.cap);
