@@ -2489,15 +2489,21 @@ __host__ __device__ inline size_t bm_sweep_lds_bytes(u32 k, u32 groups) {
 // gcache, gain1, icache (n each) and the k block weights, then u32 work,
 // dest, dcache, mark, oldp, gid (n each), then the u8 locks and stale bytes
 __host__ __device__ constexpr size_t bm_state_bytes(u32 n, u32 k) { return 50ull * n + 8ull * k; }
+// the u32 row ids, then the u8 row flags
+__host__ __device__ constexpr size_t bm_rows_bytes(u32 n) { return 5ull * n; }
 
 // LS: the per-vertex state lives in LDS (from byte state_off of the dynamic
 // LDS) for the whole launch: copied in at the start and back to device
 // memory before the kernel returns (the host's global resolve path and the
 // next launch read it there).  The device functions reach it through the
 // same pointers in a local copy of the Jet state, as generic addresses.
-// The control block is kept in LDS the same way in both forms.
+// The control block is kept in LDS the same way in both forms.  With
+// rows_off (LS only, 0: off) the BlockConn row ids and flags of the level's
+// vertices follow at that byte: the apply and row steps read them for every
+// neighbour of a move; the flags go back to device memory at the end.
 template<int G, bool LS>
-__global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters, u32 groups, u32 state_off) {
+__global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters, u32 groups, u32 state_off,
+                                                       u32 rows_off) {
     extern __shared__ __align__(16) char smem[];
     constexpr u32 CW = (sizeof(JetCtl) + 7) / 8;
     static_assert(CW <= BM && sizeof(JetCtl) % 8 == 0, "control block copy");
@@ -2534,6 +2540,16 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             d.lock[n + v] = dg.lock[n + v];  // stale bytes
         }
         for (u32 b = threadIdx.x; b < k; b += BM) d.bw[b] = dg.bw[b];
+        if (rows_off) {
+            u32 *wid = (u32 *) (smem + rows_off);
+            u8 *rf = (u8 *) (wid + n);
+            for (u32 v = threadIdx.x; v < n; v += BM) {
+                wid[v] = dg.wid[v];
+                rf[v] = dg.rflag[v];
+            }
+            d.wid = wid;
+            d.rflag = rf;
+        }
     }
     __syncthreads();
     __shared__ u32 s_live, s_mode, s_stamp, s_save;
@@ -2834,6 +2850,7 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             dg.gid[v] = d.gid[v];
             dg.lock[v] = d.lock[v];
             dg.lock[dg.n + v] = d.lock[dg.n + v];
+            if (rows_off) dg.rflag[v] = d.rflag[v];
         }
         for (u32 b = threadIdx.x; b < dg.k; b += BM) dg.bw[b] = d.bw[b];
     }
@@ -3029,6 +3046,7 @@ struct JetHost {
     size_t bm_lds = 0;
     bool bm_ls = false;  // single-workgroup form with the per-vertex state in LDS
     u32 bm_state_off = 0;
+    u32 bm_rows_off = 0;  // and the row ids and flags after it (0: in device memory)
     size_t sweep_smem = 0, hub_smem = 0;
     s64 *H = nullptr, *roff = nullptr;
     u32 *over_flag = nullptr, *olist = nullptr, *olen = nullptr;
@@ -3088,11 +3106,12 @@ struct JetHost {
     template<int GG>
     void launch_block(u32 max_iters) {
         if (bm_ls)
-            launch(c, "jet_block", 0.0, dim3(1), dim3(BM), bm_state_off + bm_state_bytes(d.n, d.k),
-                   jet_block_kernel<GG, true>, d, max_iters, bm_groups, bm_state_off);
+            launch(c, "jet_block", 0.0, dim3(1), dim3(BM),
+                   bm_rows_off ? bm_rows_off + bm_rows_bytes(d.n) : bm_state_off + bm_state_bytes(d.n, d.k),
+                   jet_block_kernel<GG, true>, d, max_iters, bm_groups, bm_state_off, bm_rows_off);
         else
             launch(c, "jet_block", 0.0, dim3(1), dim3(BM), bm_lds, jet_block_kernel<GG, false>, d, max_iters, bm_groups,
-                   0u);
+                   0u, 0u);
     }
 
     void iteration() {
@@ -3342,6 +3361,12 @@ JetResult jet_refine(Ctx &c, const DGraph &g, const GraphMeta &meta, u32 *part, 
         if (d.W <= 2) {  // counts of the counted vertices; nw * k <= m (a wide vertex has k edges)
             d.wcnt = bs.get<u32>((u64) (nw ? nw : 1) * k);
             fill<u32>(c, d.wcnt, (u64) (nw ? nw : 1) * k, 0u);
+        }
+        // the single-workgroup form keeps the row ids and flags in LDS too if
+        // they fit beside its state
+        if (h.block_mode && h.bm_ls) {
+            const size_t off = (h.bm_state_off + bm_state_bytes(n, k) + 15) & ~(size_t) 15;
+            if (off + bm_rows_bytes(n) + jet_block_static_lds(h.Gb, true) <= c.max_lds) h.bm_rows_off = (u32) off;
         }
     }
     copy_d2d(c, d.work, part, 4ull * n);
