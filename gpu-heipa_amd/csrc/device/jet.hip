@@ -2632,16 +2632,29 @@ __global__ __launch_bounds__(BM) void jet_block_kernel(JetDev dg, u32 max_iters,
             }
             __syncthreads();
             const u32 nl = s_nlist;
+            // each round loads the adjacency offsets of the group's vertex of
+            // the next round, so that they arrive with this round's edges
+            u32 nxu = 0, nxb = 0, nxe = 0;
+            if (ingroup && gi < nl) {
+                nxu = list[gi];
+                nxb = d.xadj[nxu];
+                nxe = d.xadj[nxu + 1];
+            }
             for (u32 r0 = 0; r0 < nl; r0 += groups) {
                 const u32 idx = r0 + gi;
                 const bool active = ingroup && idx < nl;
-                const u32 u = active ? list[idx] : 0;
+                const u32 u = active ? nxu : 0, ub = nxb, ue = nxe;
+                if (ingroup && idx + groups < nl) {
+                    nxu = list[idx + groups];
+                    nxb = d.xadj[nxu];
+                    nxe = d.xadj[nxu + 1];
+                }
                 if (active) {
                     if (lane == 0) {
                         nev++;
-                        degs += degree(d, u);
+                        degs += ue - ub;
                     }
-                    conn_accumulate<group_unroll<G>()>(d, gc, u, lane, G);
+                    conn_accumulate_range<group_unroll<G>()>(d, gc, ub, ue, lane, G);
                 }
                 wave_sync();
                 const u32 own = active ? d.work[u] : 0;
