@@ -26,6 +26,17 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= exported
 
 
+def test_binding_constants_match_header():
+    """Every status code, option and job kind that include/heipa.h defines
+    has the same value in the Python binding."""
+    import re
+    text = open(os.path.join(ROOT, "include", "heipa.h")).read()
+    defs = dict(re.findall(r"^#define HEIPA_((?:OK|ERR_|OPT_|JOB_)\w*)\s+(\d+)", text, re.M))
+    assert "OPT_REGION_EDGES" in defs and "ERR_IO" in defs and "JOB_HM" in defs
+    for name, value in defs.items():
+        assert getattr(heipa, name) == int(value), name
+
+
 def test_null_context_is_an_error_not_a_crash():
     L = heipa.lib()
     assert L.heipa_edge_cut(None, None, None, None) == heipa.ERR_INVALID
