@@ -958,13 +958,12 @@ __global__ __launch_bounds__(JB) void hub_sweep_kernel(JetDev d, const u32 *hubs
 // afterburner (jet:269-328): selected candidates are locked, stamped and
 // appended to the move list
 template<u32 ADJ_UNROLL>
-__device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 lane, u32 stride) {
+__device__ __forceinline__ s64 afterburner_change_range(const JetDev &d, u32 u, u64 b, u64 e, u32 lane, u32 stride) {
     const s64 ug = d.gain1[u];
     const u32 o = d.work[u], nn = d.dest[u];
     const s32 *Do = d.D ? d.D + (u64) o * d.k : nullptr, *Dn = d.D ? d.D + (u64) nn * d.k : nullptr;
     s64 change = 0;
-    const u64 e = d.xadj[u + 1];
-    for (u64 j0 = d.xadj[u] + lane; j0 < e; j0 += (u64) stride * ADJ_UNROLL) {
+    for (u64 j0 = b + lane; j0 < e; j0 += (u64) stride * ADJ_UNROLL) {
         u32 v[ADJ_UNROLL], vn[ADJ_UNROLL], vo[ADJ_UNROLL];
         s64 vg[ADJ_UNROLL], w[ADJ_UNROLL];
 #pragma unroll
@@ -993,25 +992,42 @@ __device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 la
     }
     return change;
 }
+template<u32 ADJ_UNROLL>
+__device__ __forceinline__ s64 afterburner_change(const JetDev &d, u32 u, u32 lane, u32 stride) {
+    return afterburner_change_range<ADJ_UNROLL>(d, u, d.xadj[u], d.xadj[u + 1], lane, stride);
+}
 
 // afterburner over `len` candidates of `region`, G lanes each; selected
-// vertices go to `mregion` (LDS counter); per-thread statistics in degs / gh
+// vertices go to `mregion` (LDS counter); per-thread statistics in degs / gh.
+// Each round loads the group's next candidate and its adjacency offsets, so
+// that they arrive with this round's edges.
 template<int G>
 __device__ __forceinline__ void afterburner_list(const JetDev &d, u32 stamp, const u32 *region, u32 len, u32 *mregion,
                                                  u32 *s_cnt, u64 &degs, u64 &gh) {
     const bool do_hash = d.trace != nullptr;
     const u32 groups = blockDim.x / G;
     const u32 gi = threadIdx.x / G, lane = threadIdx.x % G;
+    u32 nxu = 0, nxb = 0, nxe = 0;
+    if (gi < len) {
+        nxu = region[gi];
+        nxb = d.xadj[nxu];
+        nxe = d.xadj[nxu + 1];
+    }
     for (u32 i0 = 0; i0 < len; i0 += groups) {
         const u32 i = i0 + gi;
         const bool active = i < len;
-        const u32 u = active ? region[i] : 0;
-        const bool hub = active && degree(d, u) > d.hub_deg;
-        s64 ch = (active && !hub) ? afterburner_change<group_unroll<G>()>(d, u, lane, G) : 0;
+        const u32 u = active ? nxu : 0, ub = nxb, ue = nxe;
+        if (i + groups < len) {
+            nxu = region[i + groups];
+            nxb = d.xadj[nxu];
+            nxe = d.xadj[nxu + 1];
+        }
+        const bool hub = active && ue - ub > d.hub_deg;
+        s64 ch = (active && !hub) ? afterburner_change_range<group_unroll<G>()>(d, u, ub, ue, lane, G) : 0;
         ch = group_sum<G>(ch, (s64 *) nullptr);
         bool sel = false;
         if (active && !hub && lane == 0) {
-            degs += degree(d, u);
+            degs += ue - ub;
             const s64 ug = d.gain1[u];
             if (do_hash) gh += mix64d(mix64d(u) + (u64) ug);
             if (ug + ch >= 0) {
