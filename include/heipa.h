@@ -237,8 +237,9 @@ int heipa_pool_set_option(heipa_pool *pool, int option, uint64_t value);
  * the sample jobs through the pool once per candidate setting of three options that change launch
  * shapes but never results -- HEIPA_OPT_GROUP_CAP in {64, 16, 8}, HEIPA_OPT_BLOCK_GROUP_CAP in
  * {16, 8, 4}, HEIPA_OPT_BLOCK_JET in {1024, 2048, 4096} -- one option at a time starting from the
- * current setting, takes a candidate when the sample runs at least 2% faster with it, and leaves the
- * chosen setting on every worker (call while no job runs).  The sample's outputs are those of its
+ * current setting (each value other than the option's starting one runs once: 2 + 6 runs from the
+ * defaults), takes a candidate when the sample runs at least 2% faster with it than the best so far,
+ * and leaves the chosen setting on every worker (call while no job runs).  The sample's outputs are those of its
  * last run; a failed sample job ends the tuning with its status. */
 typedef struct {
     uint32_t group_cap, block_group_cap, block_jet;  /* the chosen setting */
